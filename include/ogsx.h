@@ -1,0 +1,226 @@
+/*
+ * ogsx.h -- C ABI of the MI355X-native implicit FEM step for OGS-5's
+ * LIQUID_FLOW and HEAT_TRANSPORT processes (element assembly + Krylov solve).
+ *
+ * The entry points replace, one for one, the calls that OGS-5's
+ * CRFProcess::Execute (FEM/rf_pcs.cpp:4695-4853) makes on the path:
+ *
+ *   CreateEQS_LinearSolver -> SparseTable/CSparseMatrix  ogsx_pattern_create,
+ *                                                       ogsx_system_create
+ *   Linear_EQS::ConfigNumerics                           ogsx_solver_config
+ *   Linear_EQS::Initialize                               ogsx_system_initialize
+ *   GlobalAssembly -> CFiniteElementStd::Assembly        ogsx_assemble
+ *   IncorporateSourceTerms                               ogsx_add_rhs
+ *   IncorporateBoundaryConditions -> SetKnownX_i         ogsx_set_dirichlet
+ *   Linear_EQS::Solver                                   ogsx_solve
+ *   Linear_EQS::Write_BIN                                ogsx_write_bin
+ *
+ * Conventions: every function returns int status (0 = OK, < 0 = error; the
+ * message is in ogsx_last_error()); no exception crosses the ABI; pointers
+ * are host pointers unless a name says _dev; sizes are int64_t.  A handle is
+ * owned by its creator and freed with its *_destroy.  Each system owns one
+ * HIP stream; calls on one system are not thread-safe, distinct systems are
+ * independent.  All device memory of a system is allocated at creation.
+ */
+#ifndef OGSX_H
+#define OGSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define OGSX_OK 0
+#define OGSX_EINVAL (-1)  /* bad argument                                  */
+#define OGSX_EHIP (-2)    /* HIP runtime error or no device                */
+#define OGSX_ENOMEM (-3)  /* device/host allocation failed                 */
+#define OGSX_ESTATE (-4)  /* call out of order (e.g. solve before assemble) */
+
+/* element types: MshElemType::type numbering (MSH/MshElemType.h) */
+#define OGSX_LINE2 1
+#define OGSX_QUAD4 2
+#define OGSX_HEX8 3
+#define OGSX_TET4 5
+
+/* processes (FiniteElement::ProcessType subset, FEM/FEMEnums.h) */
+#define OGSX_LIQUID_FLOW 1
+#define OGSX_HEAT_TRANSPORT 3
+
+/* Krylov methods: the $LINEAR_SOLVER method ids of Linear_EQS
+ * (FEM/equation_class.cpp:191-250) */
+#define OGSX_BICGSTAB 2
+#define OGSX_CG 5
+/* preconditioners (:260-292): 1 Jacobi; 100 (ILU) falls back to Jacobi */
+#define OGSX_PRECOND_NONE 0
+#define OGSX_PRECOND_JACOBI 1
+
+/* matrix storage/arithmetic of a system */
+#define OGSX_FP64 0     /* FP64 matrix, FP64 Krylov (the reference's)         */
+#define OGSX_FP32_IR 1  /* FP64 matrix for residuals/BC + FP32 copy for the
+                           inner Krylov solve inside FP64 iterative refinement */
+
+/* Dirichlet elimination modes */
+#define OGSX_DIRICHLET_ROW 0  /* CSparseMatrix::Diagonize (matrix_class.cpp:1663):
+                                 zero the row except the diagonal,
+                                 b_i = a_ii * g; the matrix turns nonsymmetric */
+#define OGSX_DIRICHLET_SYMMETRIC 1 /* additionally zero the column and lift
+                                 b_k -= a_ki * g (the colDEBUG branch :1691);
+                                 same solution, keeps A symmetric for CG      */
+
+/* vectors of a system */
+#define OGSX_VEC_X 0      /* Linear_EQS::x (initial guess in, solution out)  */
+#define OGSX_VEC_B 1      /* Linear_EQS::b                                   */
+#define OGSX_VEC_UOLD 2   /* node values at the old time level (idx0)        */
+
+typedef struct ogsx_pattern ogsx_pattern;
+typedef struct ogsx_system ogsx_system;
+
+const char* ogsx_last_error(void);
+/* Number of visible GPUs (0 on a machine without one).  No other call. */
+int ogsx_device_count(int* count);
+/* Library build string. */
+const char* ogsx_version(void);
+
+/* ---- CRS pattern ------------------------------------------------------ */
+/* Builds the sparsity pattern exactly as CFEMesh::ConnectedNodes
+ * (MSH/msh_mesh.cpp:2931-2979) + SparseTable CRS (FEM/matrix_class.cpp:
+ * 743-777): row i holds the distinct nodes of all elements touching node i
+ * in ascending order, the equation index being the node index.  Also builds
+ * the element colouring (no two elements of one colour share a node) and the
+ * element -> CRS slot map used by the atomic-free scatter.  Host work, done
+ * once per mesh.  conn: n_elem * npe node indices. */
+int ogsx_pattern_create(int64_t n_nodes, int64_t n_elem, int etype,
+                        const int64_t* conn, ogsx_pattern** out);
+int ogsx_pattern_info(const ogsx_pattern* p, int64_t* n_rows, int64_t* nnz,
+                      int* n_colors);
+/* num_column_entries[n+1], entry_column[nnz], diag_entry[n] as the reference's
+ * `long` arrays (diag = -1 for a node with no element). */
+int ogsx_pattern_export(const ogsx_pattern* p, int64_t* row_ptr, int64_t* col,
+                        int64_t* diag);
+/* Element colour of every element (n_elem ints). */
+int ogsx_pattern_colors(const ogsx_pattern* p, int32_t* color);
+int ogsx_pattern_destroy(ogsx_pattern* p);
+
+/* ---- system: Linear_EQS + the element data it is assembled from -------- */
+/* Linear_EQS::Linear_EQS (FEM/equation_class.cpp:85): device A (CRS values
+ * on the pattern), b, x and Krylov buffers; bNorm starts at 1.0. */
+int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out);
+int ogsx_system_destroy(ogsx_system* s);
+
+/* Element data for CFiniteElementStd::Assembly (FEM/fem_ele_std.cpp:9367).
+ * Uploaded once; later assemblies reuse it. */
+typedef struct {
+    int process;          /* OGSX_LIQUID_FLOW / OGSX_HEAT_TRANSPORT           */
+    int dim;              /* problem dimension (size of the K tensor)         */
+    int n_gauss;          /* $ELE_GAUSS_POINTS, default 3 (rf_num_new.cpp:124) */
+    int cond_stride;      /* 1: isotropic scalar per element; dim*dim: tensor */
+    const double* x;      /* node coordinates [n_nodes]                       */
+    const double* y;
+    const double* z;
+    const double* mass_coef; /* [n_elem]: LIQUID CalCoefMass storage
+                                (fem_ele_std.cpp:1580-1622); HEAT HeatCapacity
+                                rho*c (:1693-1697), divided by tuf            */
+    const double* cond;   /* [n_elem*cond_stride]: LIQUID permeability tensor
+                             (rf_mmp_new.cpp:4517); HEAT conductivity tensor  */
+    double tuf;           /* time_unit_factor                                 */
+    double mu;            /* LIQUID viscosity divisor (1 for the HEAD form)   */
+    double geo_area;      /* MMP geo_area, multiplies |J| for dim != 3 elems  */
+    double adv_coef;      /* HEAT: c_f*rho_f (CalCoefAdvection :3633)         */
+    const double* vel;    /* HEAT: Gauss-point velocity [n_elem][3][ngp]
+                             (ElementValue::Velocity, fem_ele_std.cpp:10309)
+                             or NULL for no advection                         */
+    double rho_g;         /* LIQUID gravity density*g (Assemble_Gravity
+                             :5859); 0 switches the term off                  */
+} ogsx_elements;
+int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e);
+
+/* Linear_EQS::Initialize (equation_class.cpp:325): A = 0, b = 0. */
+int ogsx_system_initialize(ogsx_system* s);
+
+/* GlobalAssembly's element loop (rf_pcs.cpp:5738-5763) over every element:
+ * CalcMass/CalcLaplace/CalcAdvection at the Gauss points, the parabolic
+ * (LIQUID, theta = nls_relaxation) or mixed hyperbolic-parabolic (HEAT,
+ * theta = ls_theta) time discretisation, gravity, and the scatter of the
+ * local LHS into A and RHS into b.  steady != 0 drops the mass matrix (HEAT
+ * tim_type STEADY, fem_ele_std.cpp:8543; for LIQUID the same operator as a
+ * zero storage coefficient).  u_old is the system's OGSX_VEC_UOLD vector. */
+int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady);
+
+/* IncorporateSourceTerms (rf_pcs.cpp:8002; point sources :8447):
+ * b[idx[k]] += val[k] in list order. */
+int ogsx_add_rhs(ogsx_system* s, int64_t n, const int64_t* idx,
+                 const double* val);
+
+/* IncorporateBoundaryConditions -> Linear_EQS::SetKnownX_i for every
+ * Dirichlet node (rf_pcs.cpp:6991, equation_class.cpp:926). */
+int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx,
+                       const double* val, int mode);
+
+/* Linear_EQS::ConfigNumerics + Solver (equation_class.cpp:191, :857). */
+typedef struct {
+    int method;        /* OGSX_CG or OGSX_BICGSTAB                            */
+    int precond;       /* OGSX_PRECOND_JACOBI (100 accepted) or NONE          */
+    int max_iter;      /* ls_max_iterations                                   */
+    double tol;        /* ls_error_tolerance: ||r||/||b|| < tol               */
+    /* OGSX_FP32_IR systems only */
+    double inner_tol;  /* relative reduction asked of each FP32 inner solve   */
+    int max_outer;     /* refinement steps                                    */
+} ogsx_solver_config;
+
+typedef struct {
+    int iterations;    /* Krylov iterations (inner iterations summed for IR)  */
+    int outer;         /* refinement steps (IR), else 0                       */
+    int converged;     /* 1 converged, 0 max_iter reached, 2 skipped by the
+                          CheckNormRHS test (equation_class.cpp:1104),
+                          3 breakdown (rho or omega below DBL_MIN)            */
+    double error;      /* ||r||/||b|| at exit (recurrence residual for FP64;
+                          true FP64 residual for IR)                          */
+    double bnorm;      /* ||b|| (persists as the next solve's bNorm)          */
+    int ret;           /* Linear_EQS::Solver's own return value: BiCGStab the
+                          iteration count, CG 1/0 (iter <= max_iter)          */
+} ogsx_solve_info;
+
+/* Solves A x = b from the current x.  FP64 systems run the reference's
+ * recurrences (Jacobi-CG :1129, right-preconditioned BiCGStab :1287) on the
+ * device; OGSX_FP32_IR systems run them in FP32 inside FP64 iterative
+ * refinement (r = b - A x, norms and the stopping test in FP64). */
+int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg,
+               ogsx_solve_info* info);
+
+/* Host <-> device copies of OGSX_VEC_* vectors (n_rows doubles). */
+int ogsx_set_vector(ogsx_system* s, int which, const double* host);
+int ogsx_get_vector(ogsx_system* s, int which, double* host);
+/* Device-side copy between vectors (the time-level shift u_old <- x of
+ * CRFProcess's copy-back, rf_pcs.cpp:5054-5090). */
+int ogsx_copy_vector(ogsx_system* s, int dst, int src);
+/* CRS values of A (nnz doubles, row-major CRS order) to the host. */
+int ogsx_get_matrix(ogsx_system* s, double* values);
+/* y = A x on the device with host vectors (CSparseMatrix::multiVec,
+ * matrix_class.cpp:1467); for parity tests of the SpMV kernel alone. */
+int ogsx_spmv(ogsx_system* s, const double* x, double* y);
+/* Linear_EQS::Write_BIN (equation_class.cpp:409; matrix_class.cpp:1403):
+ * int64 n; int64 ptr[n+1]; int64 col[nnz]; f64 A[nnz]; f64 b[n]. */
+int ogsx_write_bin(ogsx_system* s, const char* path);
+
+/* ---- measurement ------------------------------------------------------ */
+/* Enables HIP-event timing of every SpMV launch that ogsx_solve issues (on
+ * the system's stream) and returns the running totals. */
+int ogsx_profile_enable(ogsx_system* s, int on);
+int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
+                      int64_t* launches, double* bytes_per_launch);
+#define OGSX_PROF_SPMV64 0  /* FP64 SpMV kernel                              */
+#define OGSX_PROF_SPMV32 1  /* FP32 SpMV kernel (IR inner solve)             */
+#define OGSX_PROF_ASSEMBLY 2
+/* Chooses the SpMV lanes-per-row variant: 0 = autotune on first use,
+ * otherwise 4, 8, 16, 32 or 64. */
+int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row);
+int ogsx_get_spmv_variant(ogsx_system* s, int* lanes_per_row);
+/* Synchronises the system's stream. */
+int ogsx_synchronize(ogsx_system* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

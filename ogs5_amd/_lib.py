@@ -1,0 +1,275 @@
+"""ctypes binding of the C ABI in include/ogsx.h (libogsx.so, built in-tree).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libogsx.so")
+
+OK = 0
+LINE2, QUAD4, HEX8, TET4 = 1, 2, 3, 5
+LIQUID_FLOW, HEAT_TRANSPORT = 1, 3
+BICGSTAB, CG = 2, 5
+PRECOND_NONE, PRECOND_JACOBI = 0, 1
+FP64, FP32_IR = 0, 1
+DIRICHLET_ROW, DIRICHLET_SYMMETRIC = 0, 1
+VEC_X, VEC_B, VEC_UOLD = 0, 1, 2
+PROF_SPMV64, PROF_SPMV32, PROF_ASSEMBLY = 0, 1, 2
+
+
+class OgsxError(RuntimeError):
+    pass
+
+
+class Elements(C.Structure):
+    _fields_ = [
+        ("process", C.c_int), ("dim", C.c_int), ("n_gauss", C.c_int), ("cond_stride", C.c_int),
+        ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+        ("mass_coef", C.c_void_p), ("cond", C.c_void_p),
+        ("tuf", C.c_double), ("mu", C.c_double), ("geo_area", C.c_double),
+        ("adv_coef", C.c_double), ("vel", C.c_void_p), ("rho_g", C.c_double),
+    ]
+
+
+class SolverConfig(C.Structure):
+    _fields_ = [("method", C.c_int), ("precond", C.c_int), ("max_iter", C.c_int),
+                ("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("converged", C.c_int),
+                ("error", C.c_double), ("bnorm", C.c_double), ("ret", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Every exported symbol and its signature (mirrors include/ogsx.h).
+_P = C.c_void_p
+_I64 = C.c_int64
+SIGNATURES = {
+    "ogsx_last_error": (C.c_char_p, []),
+    "ogsx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "ogsx_version": (C.c_char_p, []),
+    "ogsx_pattern_create": (C.c_int, [_I64, _I64, C.c_int, _P, C.POINTER(_P)]),
+    "ogsx_pattern_info": (C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int)]),
+    "ogsx_pattern_export": (C.c_int, [_P, _P, _P, _P]),
+    "ogsx_pattern_colors": (C.c_int, [_P, _P]),
+    "ogsx_pattern_destroy": (C.c_int, [_P]),
+    "ogsx_system_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "ogsx_system_destroy": (C.c_int, [_P]),
+    "ogsx_system_set_elements": (C.c_int, [_P, C.POINTER(Elements)]),
+    "ogsx_system_initialize": (C.c_int, [_P]),
+    "ogsx_assemble": (C.c_int, [_P, C.c_double, C.c_double, C.c_int]),
+    "ogsx_add_rhs": (C.c_int, [_P, _I64, _P, _P]),
+    "ogsx_set_dirichlet": (C.c_int, [_P, _I64, _P, _P, C.c_int]),
+    "ogsx_solve": (C.c_int, [_P, C.POINTER(SolverConfig), C.POINTER(SolveInfo)]),
+    "ogsx_set_vector": (C.c_int, [_P, C.c_int, _P]),
+    "ogsx_get_vector": (C.c_int, [_P, C.c_int, _P]),
+    "ogsx_copy_vector": (C.c_int, [_P, C.c_int, C.c_int]),
+    "ogsx_get_matrix": (C.c_int, [_P, _P]),
+    "ogsx_spmv": (C.c_int, [_P, _P, _P]),
+    "ogsx_write_bin": (C.c_int, [_P, C.c_char_p]),
+    "ogsx_profile_enable": (C.c_int, [_P, C.c_int]),
+    "ogsx_profile_read": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(_I64),
+                                    C.POINTER(C.c_double)]),
+    "ogsx_set_spmv_variant": (C.c_int, [_P, C.c_int]),
+    "ogsx_get_spmv_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "ogsx_synchronize": (C.c_int, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libogsx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OgsxError(f"{LIB_PATH} is missing: run __graft_entry__.build() first "
+                            "(the HIP library has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != OK:
+        msg = lib().ogsx_last_error().decode()
+        raise OgsxError(f"{what} failed ({rc}): {msg}")
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    check(lib().ogsx_device_count(C.byref(c)), "ogsx_device_count")
+    return c.value
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class Pattern:
+    """CRS sparsity pattern + element colouring (ogsx_pattern_create)."""
+
+    def __init__(self, n_nodes: int, etype: int, conn: np.ndarray):
+        conn = _i64(conn)
+        self._keep = conn
+        h = C.c_void_p()
+        n_elem = conn.shape[0] if conn.ndim == 2 else conn.size // {1: 2, 2: 4, 3: 8, 5: 4}[etype]
+        check(lib().ogsx_pattern_create(int(n_nodes), int(n_elem), int(etype), _ptr(conn),
+                                        C.byref(h)), "ogsx_pattern_create")
+        self.h = h
+        self.etype = etype
+        self.n_elem = int(n_elem)
+        n, nnz, nc = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().ogsx_pattern_info(h, C.byref(n), C.byref(nnz), C.byref(nc)), "info")
+        self.n, self.nnz, self.n_colors = n.value, nnz.value, nc.value
+
+    def export(self):
+        ptr = np.empty(self.n + 1, np.int64)
+        col = np.empty(self.nnz, np.int64)
+        diag = np.empty(self.n, np.int64)
+        check(lib().ogsx_pattern_export(self.h, _ptr(ptr), _ptr(col), _ptr(diag)), "export")
+        return ptr, col, diag
+
+    def colors(self):
+        c = np.empty(self.n_elem, np.int32)
+        check(lib().ogsx_pattern_colors(self.h, _ptr(c)), "colors")
+        return c
+
+    def close(self):
+        if self.h:
+            lib().ogsx_pattern_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class System:
+    """One device-resident linear system (ogsx_system_*)."""
+
+    def __init__(self, pattern: Pattern, precision: int = FP64):
+        self.pattern = pattern
+        h = C.c_void_p()
+        check(lib().ogsx_system_create(pattern.h, int(precision), C.byref(h)), "ogsx_system_create")
+        self.h = h
+        self.n = pattern.n
+        self.nnz = pattern.nnz
+        self.precision = precision
+        self._keep = []
+
+    def set_elements(self, *, process, dim, x, y=None, z=None, mass_coef, cond, cond_stride=1,
+                     n_gauss=3, tuf=1.0, mu=1.0, geo_area=1.0, adv_coef=0.0, vel=None,
+                     rho_g=0.0):
+        arrs = [_f64(x), _f64(y) if y is not None else None, _f64(z) if z is not None else None,
+                _f64(mass_coef), _f64(cond), _f64(vel) if vel is not None else None]
+        self._keep = arrs
+        e = Elements(process, dim, n_gauss, cond_stride,
+                     *(None if a is None else _ptr(a).value for a in arrs[:5]),
+                     tuf, mu, geo_area, adv_coef,
+                     None if arrs[5] is None else _ptr(arrs[5]).value, rho_g)
+        check(lib().ogsx_system_set_elements(self.h, C.byref(e)), "ogsx_system_set_elements")
+
+    def initialize(self):
+        check(lib().ogsx_system_initialize(self.h), "ogsx_system_initialize")
+
+    def assemble(self, dt, theta, steady=False):
+        check(lib().ogsx_assemble(self.h, float(dt), float(theta), int(bool(steady))),
+              "ogsx_assemble")
+
+    def add_rhs(self, idx, val):
+        idx, val = _i64(idx), _f64(val)
+        check(lib().ogsx_add_rhs(self.h, idx.size, _ptr(idx), _ptr(val)), "ogsx_add_rhs")
+
+    def set_dirichlet(self, idx, val, mode=DIRICHLET_ROW):
+        idx, val = _i64(idx), _f64(val)
+        check(lib().ogsx_set_dirichlet(self.h, idx.size, _ptr(idx), _ptr(val), int(mode)),
+              "ogsx_set_dirichlet")
+
+    def solve(self, method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
+              inner_tol=1e-4, max_outer=50) -> dict:
+        cfg = SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer)
+        info = SolveInfo()
+        check(lib().ogsx_solve(self.h, C.byref(cfg), C.byref(info)), "ogsx_solve")
+        return info.as_dict()
+
+    def set_vector(self, which, v):
+        v = _f64(v)
+        assert v.size == self.n
+        check(lib().ogsx_set_vector(self.h, which, _ptr(v)), "ogsx_set_vector")
+
+    def get_vector(self, which) -> np.ndarray:
+        v = np.empty(self.n, np.float64)
+        check(lib().ogsx_get_vector(self.h, which, _ptr(v)), "ogsx_get_vector")
+        return v
+
+    def copy_vector(self, dst, src):
+        check(lib().ogsx_copy_vector(self.h, dst, src), "ogsx_copy_vector")
+
+    def get_matrix(self) -> np.ndarray:
+        a = np.empty(self.nnz, np.float64)
+        check(lib().ogsx_get_matrix(self.h, _ptr(a)), "ogsx_get_matrix")
+        return a
+
+    def spmv(self, x) -> np.ndarray:
+        x = _f64(x)
+        y = np.empty(self.n, np.float64)
+        check(lib().ogsx_spmv(self.h, _ptr(x), _ptr(y)), "ogsx_spmv")
+        return y
+
+    def write_bin(self, path):
+        check(lib().ogsx_write_bin(self.h, str(path).encode()), "ogsx_write_bin")
+
+    def profile(self, on=True):
+        check(lib().ogsx_profile_enable(self.h, int(on)), "ogsx_profile_enable")
+
+    def profile_read(self, kind):
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        check(lib().ogsx_profile_read(self.h, kind, C.byref(ms), C.byref(n), C.byref(b)),
+              "ogsx_profile_read")
+        return ms.value, n.value, b.value
+
+    def set_spmv_variant(self, lanes_per_row):
+        check(lib().ogsx_set_spmv_variant(self.h, int(lanes_per_row)), "ogsx_set_spmv_variant")
+
+    def spmv_variant(self) -> int:
+        v = C.c_int()
+        check(lib().ogsx_get_spmv_variant(self.h, C.byref(v)), "ogsx_get_spmv_variant")
+        return v.value
+
+    def synchronize(self):
+        check(lib().ogsx_synchronize(self.h), "ogsx_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ogsx_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,930 @@
+// api.hip -- host runtime behind include/ogsx.h: device allocation, element
+// upload, assembly launch per colour, boundary conditions, and the Krylov
+// drivers (FP64 and FP32-inside-FP64 iterative refinement).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "ogsx_internal.h"
+
+using namespace ogsx;
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(OGSX_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int grid_for(int64_t n)
+{
+    int64_t g = (n + 255) / 256;
+    g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+    return (int)std::max<int64_t>(kNumXcd, std::min<int64_t>(g, 2048));
+}
+
+int blocks_for(int64_t n, int per)
+{
+    int64_t b = (n + per - 1) / per;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 65535));
+}
+}  // namespace
+
+struct ogsx_pattern {
+    PatternHost h;
+};
+
+template <typename T>
+struct DevVecs {
+    T *x = nullptr, *b = nullptr, *d = nullptr, *r = nullptr, *p = nullptr, *s = nullptr,
+      *z = nullptr, *r0 = nullptr, *v = nullptr, *t = nullptr, *ph = nullptr, *sh = nullptr;
+    KVec<T> kv() const { return KVec<T>{x, b, d, r, p, s, z, r0, v, t, ph, sh}; }
+};
+
+struct ProfAcc {
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct ogsx_system {
+    const PatternHost* pat = nullptr;
+    int precision = OGSX_FP64;
+    int n = 0;
+    int64_t nnz = 0;
+    int G = 8;
+    int lpr = 0;  // SpMV lanes per row (0 = choose on first use)
+    hipStream_t stream = nullptr;
+    // pattern on device
+    int *ptr = nullptr, *col = nullptr, *diagpos = nullptr;
+    int *conn = nullptr, *color_elems = nullptr, *slot = nullptr;
+    // values
+    double* A = nullptr;
+    float* A32 = nullptr;
+    double* uold = nullptr;
+    DevVecs<double> v64;
+    DevVecs<float> v32;
+    double* r64 = nullptr;  // IR residual
+    double* part = nullptr; // partial sums [4][G] x 3 sets
+    double* red = nullptr;  // reduced scalars
+    KrylovState* st = nullptr;
+    KrylovState* st_host = nullptr;  // pinned
+    double* red_host = nullptr;      // pinned
+    // BC / ST
+    int8_t* bcflag = nullptr;
+    double* bcg = nullptr;
+    std::vector<int64_t> last_bc_idx, last_st_idx;
+    std::vector<double> last_bc_val, last_st_val;
+    int *bc_idx_d = nullptr, *st_node_d = nullptr, *st_vptr_d = nullptr;
+    double *bc_val_d = nullptr, *st_val_d = nullptr;
+    int bc_cap = 0, bc_val_cap = 0, st_cap = 0, st_nu = 0;
+    int bc_m = 0;
+    // elements
+    bool elements_set = false;
+    bool use_vel = false;
+    size_t vel_cap = 0;
+    int process = 0, dim = 0, ngp = 0, cond_stride = 1;
+    double tuf = 1, mu = 1, geo_area = 1, adv_coef = 0, rho_g = 0;
+    double *ex = nullptr, *ey = nullptr, *ez = nullptr, *mass_coef = nullptr, *cond = nullptr,
+           *vel = nullptr, *tabN = nullptr, *tabdN = nullptr, *tabW = nullptr;
+    // persistent Linear_EQS::bNorm (equation_class.cpp:135)
+    double bnorm = 1.0;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;  // (kind, pool index of start)
+    size_t ev_next = 0;
+    ProfAcc prof_acc[3];
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(ogsx_system* s, T** p, size_t count)
+{
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(OGSX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    s->allocs.push_back(q);
+    *p = (T*)q;
+    return OGSX_OK;
+}
+
+#define OK_TRY(...)                     \
+    do {                                \
+        int rc_ = (__VA_ARGS__);        \
+        if (rc_ != OGSX_OK) return rc_; \
+    } while (0)
+
+int choose_lpr(const ogsx_system* s)
+{
+    const double avg = s->n ? (double)s->nnz / s->n : 1.0;
+    int l = 4;
+    while (l < 64 && l < avg) l *= 2;
+    return l;
+}
+
+// ---- profiling helpers ----------------------------------------------------
+int prof_begin(ogsx_system* s, int kind)
+{
+    if (!s->prof) return -1;
+    if (s->ev_next + 2 > s->ev_pool.size()) {
+        // pool exhausted: drain pending pairs
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (auto& pr : s->ev_pending) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev_pool[pr.second], s->ev_pool[pr.second + 1]));
+            s->prof_acc[pr.first].ms += ms;
+            s->prof_acc[pr.first].launches++;
+        }
+        s->ev_pending.clear();
+        s->ev_next = 0;
+    }
+    int k = (int)s->ev_next;
+    s->ev_next += 2;
+    (void)hipEventRecord(s->ev_pool[k], s->stream);
+    s->ev_pending.push_back({kind, k});
+    return k;
+}
+void prof_end(ogsx_system* s, int k)
+{
+    if (k >= 0) (void)hipEventRecord(s->ev_pool[k + 1], s->stream);
+}
+int prof_flush(ogsx_system* s)
+{
+    if (!s->prof || s->ev_pending.empty()) return OGSX_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (auto& pr : s->ev_pending) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_pool[pr.second], s->ev_pool[pr.second + 1]));
+        s->prof_acc[pr.first].ms += ms;
+        s->prof_acc[pr.first].launches++;
+    }
+    s->ev_pending.clear();
+    s->ev_next = 0;
+    return OGSX_OK;
+}
+
+// ---- SpMV-family launch dispatch on lanes-per-row -------------------------
+#define LPR_DISPATCH(lpr, CALL)          \
+    switch (lpr) {                       \
+        case 4: { constexpr int L = 4; CALL; } break;   \
+        case 8: { constexpr int L = 8; CALL; } break;   \
+        case 16: { constexpr int L = 16; CALL; } break; \
+        case 32: { constexpr int L = 32; CALL; } break; \
+        default: { constexpr int L = 64; CALL; } break; \
+    }
+
+template <typename VT, typename T>
+void launch_cg_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, double tol,
+                     int prof_kind)
+{
+    const int G = s->G;
+    double* pA = s->part;
+    double* pB = s->part + 4 * G;
+    for (int it = 0; it < iters; it++) {
+        int pk = prof_begin(s, prof_kind);
+        LPR_DISPATCH(s->lpr, (k_cg_spmv<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                 s->n, s->ptr, s->col, val, dv.kv(), pA, s->st)));
+        prof_end(s, pk);
+        k_cg_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pA, pB, s->st);
+        k_cg_direction<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pB, s->st, tol);
+    }
+}
+
+template <typename VT, typename T>
+void launch_bi_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, double tol,
+                     int prof_kind)
+{
+    const int G = s->G;
+    double* pR = s->part;          // (r,r), (r0,r)
+    double* pV = s->part + 4 * G;  // (r0,v)
+    double* pS = s->part + 8 * G;  // (s,s)
+    double* pT = s->part + 12 * G; // (t,t), (t,s)
+    for (int it = 0; it < iters; it++) {
+        k_bi_dir<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pR, s->st, tol);
+        int pk = prof_begin(s, prof_kind);
+        LPR_DISPATCH(s->lpr, (k_bi_spmv_v<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                 s->n, s->ptr, s->col, val, dv.kv(), pV, s->st)));
+        prof_end(s, pk);
+        k_bi_s<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pV, pS, s->st);
+        pk = prof_begin(s, prof_kind);
+        LPR_DISPATCH(s->lpr, (k_bi_spmv_t<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                 s->n, s->ptr, s->col, val, dv.kv(), pS, pT, s->st, tol)));
+        prof_end(s, pk);
+        k_bi_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pT, pR, s->st);
+    }
+}
+
+// Runs one Krylov solve on (val, dv) from dv.x; fills st_host.
+template <typename VT, typename T>
+int krylov(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol, int max_iter,
+           double bnorm_prev, int prof_kind)
+{
+    const int G = s->G;
+    if (method == OGSX_CG) {
+        LPR_DISPATCH(s->lpr, (k_cg_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                 s->n, s->ptr, s->col, val, dv.kv(), s->part)));
+        k_cg_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, bnorm_prev, tol, max_iter);
+    } else {
+        LPR_DISPATCH(s->lpr, (k_bi_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                 s->n, s->ptr, s->col, val, dv.kv(), s->part)));
+        k_bi_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, bnorm_prev, tol, max_iter);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int chunk = 4;
+    int launched = 0;
+    while (!s->st_host->done) {
+        const int c = std::min(chunk, max_iter + 1 - launched);
+        if (c <= 0) break;
+        if (method == OGSX_CG)
+            launch_cg_chunk<VT, T>(s, val, dv, c, tol, prof_kind);
+        else
+            launch_bi_chunk<VT, T>(s, val, dv, c, tol, prof_kind);
+        launched += c;
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                               s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        OK_TRY(prof_flush(s));
+        chunk = std::min(chunk * 2, 64);
+    }
+    return OGSX_OK;
+}
+
+template <typename T>
+int alloc_vecs(ogsx_system* s, DevVecs<T>& v)
+{
+    T** f[] = {&v.x, &v.b, &v.d, &v.r, &v.p, &v.s, &v.z, &v.r0, &v.v, &v.t, &v.ph, &v.sh};
+    for (T** q : f) OK_TRY(dalloc(s, q, (size_t)s->n));
+    return OGSX_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+const char* ogsx_last_error(void) { return g_last_error.c_str(); }
+
+const char* ogsx_version(void) { return "ogsx 0.1 gfx950"; }
+
+int ogsx_device_count(int* count)
+{
+    if (!count) return fail(OGSX_EINVAL, "null count");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return OGSX_OK;
+}
+
+int ogsx_pattern_create(int64_t n_nodes, int64_t n_elem, int etype, const int64_t* conn,
+                        ogsx_pattern** out)
+{
+    if (!out) return fail(OGSX_EINVAL, "null output handle");
+    *out = nullptr;
+    auto* p = new (std::nothrow) ogsx_pattern();
+    if (!p) return fail(OGSX_ENOMEM, "host allocation failed");
+    std::string msg;
+    int rc = build_pattern(n_nodes, n_elem, etype, conn, p->h, msg);
+    if (rc != OGSX_OK) {
+        delete p;
+        return fail(rc, msg);
+    }
+    *out = p;
+    return OGSX_OK;
+}
+
+int ogsx_pattern_info(const ogsx_pattern* p, int64_t* n_rows, int64_t* nnz, int* n_colors)
+{
+    if (!p) return fail(OGSX_EINVAL, "null pattern");
+    if (n_rows) *n_rows = p->h.n;
+    if (nnz) *nnz = p->h.nnz;
+    if (n_colors) *n_colors = p->h.n_colors;
+    return OGSX_OK;
+}
+
+int ogsx_pattern_export(const ogsx_pattern* p, int64_t* row_ptr, int64_t* col, int64_t* diag)
+{
+    if (!p) return fail(OGSX_EINVAL, "null pattern");
+    if (row_ptr) std::memcpy(row_ptr, p->h.ptr.data(), sizeof(int64_t) * (p->h.n + 1));
+    if (col) std::memcpy(col, p->h.col.data(), sizeof(int64_t) * p->h.nnz);
+    if (diag) std::memcpy(diag, p->h.diag.data(), sizeof(int64_t) * p->h.n);
+    return OGSX_OK;
+}
+
+int ogsx_pattern_colors(const ogsx_pattern* p, int32_t* color)
+{
+    if (!p || !color) return fail(OGSX_EINVAL, "null argument");
+    std::memcpy(color, p->h.color.data(), sizeof(int32_t) * p->h.n_elem);
+    return OGSX_OK;
+}
+
+int ogsx_pattern_destroy(ogsx_pattern* p)
+{
+    delete p;
+    return OGSX_OK;
+}
+
+int ogsx_system_destroy(ogsx_system* s)
+{
+    if (!s) return OGSX_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void* q : s->allocs) (void)hipFree(q);
+    if (s->st_host) (void)hipHostFree(s->st_host);
+    if (s->red_host) (void)hipHostFree(s->red_host);
+    for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return OGSX_OK;
+}
+
+int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
+{
+    if (!p || !out) return fail(OGSX_EINVAL, "null argument");
+    if (precision != OGSX_FP64 && precision != OGSX_FP32_IR)
+        return fail(OGSX_EINVAL, "unknown precision");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(OGSX_EHIP, "no HIP device available");
+    auto* s = new (std::nothrow) ogsx_system();
+    if (!s) return fail(OGSX_ENOMEM, "host allocation failed");
+    s->pat = &p->h;
+    s->precision = precision;
+    s->n = (int)p->h.n;
+    s->nnz = p->h.nnz;
+    s->G = grid_for(s->n);
+    int rc = OGSX_OK;
+    auto cleanup = [&](int code) {
+        ogsx_system_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(OGSX_EHIP, "hipStreamCreate failed"));
+    const PatternHost& h = p->h;
+    std::vector<int> ptr32(h.n + 1), col32(h.nnz), dg32(h.n);
+    for (int64_t i = 0; i <= h.n; i++) ptr32[i] = (int)h.ptr[i];
+    for (int64_t k = 0; k < h.nnz; k++) col32[k] = (int)h.col[k];
+    for (int64_t i = 0; i < h.n; i++) dg32[i] = (int)h.diag[i];
+    if ((rc = dalloc(s, &s->ptr, h.n + 1)) || (rc = dalloc(s, &s->col, h.nnz)) ||
+        (rc = dalloc(s, &s->diagpos, h.n)) || (rc = dalloc(s, &s->conn, h.conn.size())) ||
+        (rc = dalloc(s, &s->color_elems, h.color_elems.size())) ||
+        (rc = dalloc(s, &s->slot, h.slot.size())) || (rc = dalloc(s, &s->A, h.nnz)) ||
+        (rc = dalloc(s, &s->uold, h.n)) || (rc = alloc_vecs(s, s->v64)) ||
+        (rc = dalloc(s, &s->part, (size_t)16 * s->G)) || (rc = dalloc(s, &s->red, 8)) ||
+        (rc = dalloc(s, &s->st, 1)) || (rc = dalloc(s, &s->bcflag, h.n)) ||
+        (rc = dalloc(s, &s->bcg, h.n)))
+        return cleanup(rc);
+    if (precision == OGSX_FP32_IR) {
+        if ((rc = dalloc(s, &s->A32, h.nnz)) || (rc = alloc_vecs(s, s->v32)) ||
+            (rc = dalloc(s, &s->r64, h.n)))
+            return cleanup(rc);
+    }
+    if (hipHostMalloc((void**)&s->st_host, sizeof(KrylovState)) != hipSuccess ||
+        hipHostMalloc((void**)&s->red_host, 8 * sizeof(double)) != hipSuccess)
+        return cleanup(fail(OGSX_ENOMEM, "hipHostMalloc failed"));
+    auto up = [&](void* d, const void* src, size_t bytes) {
+        return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s->stream);
+    };
+    if (up(s->ptr, ptr32.data(), 4 * (h.n + 1)) || up(s->col, col32.data(), 4 * h.nnz) ||
+        up(s->diagpos, dg32.data(), 4 * h.n) ||
+        up(s->conn, h.conn.data(), 4 * h.conn.size()) ||
+        up(s->color_elems, h.color_elems.data(), 4 * h.color_elems.size()) ||
+        up(s->slot, h.slot.data(), 4 * h.slot.size()))
+        return cleanup(fail(OGSX_EHIP, "upload of the pattern failed"));
+    const int nb = blocks_for(h.nnz, kBlock * 4);
+    k_fill<double><<<nb, kBlock, 0, s->stream>>>(h.nnz, s->A, 0.0);
+    double* zv[] = {s->v64.x, s->v64.b, s->uold};
+    for (double* q : zv) k_fill<double><<<blocks_for(h.n, kBlock * 4), kBlock, 0, s->stream>>>(h.n, q, 0.0);
+    k_fill<int8_t><<<blocks_for(h.n, kBlock * 4), kBlock, 0, s->stream>>>(h.n, s->bcflag, (int8_t)0);
+    if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return cleanup(fail(OGSX_EHIP, "initialisation kernels failed"));
+    s->lpr = choose_lpr(s);
+    *out = s;
+    return OGSX_OK;
+}
+
+int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e)
+{
+    if (!s || !e) return fail(OGSX_EINVAL, "null argument");
+    const PatternHost& h = *s->pat;
+    const int ed = edim_of(h.etype);
+    if (e->process != OGSX_LIQUID_FLOW && e->process != OGSX_HEAT_TRANSPORT)
+        return fail(OGSX_EINVAL, "unknown process");
+    if (e->dim != ed) return fail(OGSX_EINVAL, "problem dimension must equal the element dimension");
+    if (e->cond_stride != 1 && e->cond_stride != ed * ed)
+        return fail(OGSX_EINVAL, "cond_stride must be 1 or dim*dim");
+    if (!e->x || (ed >= 2 && !e->y) || (ed >= 3 && !e->z) || !e->mass_coef || !e->cond)
+        return fail(OGSX_EINVAL, "missing element data");
+    if (e->mu == 0.0 || e->tuf == 0.0) return fail(OGSX_EINVAL, "tuf and mu must be nonzero");
+    GaussTable t;
+    if (build_gauss_table(h.etype, e->n_gauss, t) != OGSX_OK)
+        return fail(OGSX_EINVAL, "unsupported n_gauss");
+    if (!s->elements_set) {
+        OK_TRY(dalloc(s, &s->ex, h.n));
+        OK_TRY(dalloc(s, &s->ey, h.n));
+        OK_TRY(dalloc(s, &s->ez, h.n));
+        OK_TRY(dalloc(s, &s->mass_coef, h.n_elem));
+        OK_TRY(dalloc(s, &s->cond, (size_t)h.n_elem * ed * ed));
+        OK_TRY(dalloc(s, &s->tabN, kMaxGaussPts * 8));
+        OK_TRY(dalloc(s, &s->tabdN, kMaxGaussPts * 24));
+        OK_TRY(dalloc(s, &s->tabW, kMaxGaussPts * 3));
+    }
+    s->process = e->process;
+    s->dim = e->dim;
+    s->ngp = t.ngp;
+    s->cond_stride = e->cond_stride;
+    s->tuf = e->tuf;
+    s->mu = e->mu;
+    s->geo_area = e->geo_area;
+    s->adv_coef = e->adv_coef;
+    s->rho_g = e->rho_g;
+    auto up = [&](void* d, const void* src, size_t bytes) {
+        return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s->stream);
+    };
+    HIP_TRY(up(s->ex, e->x, 8 * h.n));
+    if (ed >= 2) HIP_TRY(up(s->ey, e->y, 8 * h.n));
+    if (ed >= 3) HIP_TRY(up(s->ez, e->z, 8 * h.n));
+    HIP_TRY(up(s->mass_coef, e->mass_coef, 8 * h.n_elem));
+    HIP_TRY(up(s->cond, e->cond, 8 * (size_t)h.n_elem * e->cond_stride));
+    HIP_TRY(up(s->tabN, t.N.data(), 8 * t.N.size()));
+    HIP_TRY(up(s->tabdN, t.dN.data(), 8 * t.dN.size()));
+    HIP_TRY(up(s->tabW, t.w.data(), 8 * t.w.size()));
+    s->use_vel = (e->vel != nullptr) && e->process == OGSX_HEAT_TRANSPORT;
+    if (s->use_vel) {
+        const size_t need = (size_t)h.n_elem * 3 * t.ngp;
+        if (need > s->vel_cap) {
+            OK_TRY(dalloc(s, &s->vel, need));
+            s->vel_cap = need;
+        }
+        HIP_TRY(up(s->vel, e->vel, 8 * need));
+    }
+    s->elements_set = true;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return OGSX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int ET, int PROC, bool ISO>
+void launch_assemble_colours(ogsx_system* s, const AsmArgs& a)
+{
+    const PatternHost& h = *s->pat;
+    for (int c = 0; c < h.n_colors; c++) {
+        const int off = h.color_ptr[c], cnt = h.color_ptr[c + 1] - off;
+        if (cnt == 0) continue;
+        k_assemble<ET, PROC, ISO><<<(cnt + 63) / 64, 64, 0, s->stream>>>(a, off, cnt);
+    }
+}
+
+template <int ET>
+void launch_assemble_et(ogsx_system* s, const AsmArgs& a)
+{
+    const bool iso = s->cond_stride == 1;
+    if (s->process == OGSX_LIQUID_FLOW) {
+        if (iso)
+            launch_assemble_colours<ET, OGSX_LIQUID_FLOW, true>(s, a);
+        else
+            launch_assemble_colours<ET, OGSX_LIQUID_FLOW, false>(s, a);
+    } else {
+        if (iso)
+            launch_assemble_colours<ET, OGSX_HEAT_TRANSPORT, true>(s, a);
+        else
+            launch_assemble_colours<ET, OGSX_HEAT_TRANSPORT, false>(s, a);
+    }
+}
+
+// Copies host lists to device buffers that grow on demand.
+template <typename T>
+int upload_list(ogsx_system* s, T** dptr, int* cap, const T* src, int count)
+{
+    if (count > *cap) {
+        int newcap = std::max(count, 2 * *cap);
+        OK_TRY(dalloc(s, dptr, (size_t)newcap));
+        *cap = newcap;
+    }
+    if (count > 0)
+        HIP_TRY(hipMemcpyAsync(*dptr, src, sizeof(T) * count, hipMemcpyHostToDevice, s->stream));
+    return OGSX_OK;
+}
+
+int check_vec(ogsx_system* s, int which, double** p)
+{
+    switch (which) {
+        case OGSX_VEC_X: *p = s->v64.x; return OGSX_OK;
+        case OGSX_VEC_B: *p = s->v64.b; return OGSX_OK;
+        case OGSX_VEC_UOLD: *p = s->uold; return OGSX_OK;
+    }
+    return fail(OGSX_EINVAL, "unknown vector id");
+}
+
+}  // namespace
+
+extern "C" {
+
+int ogsx_system_initialize(ogsx_system* s)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    k_fill<double><<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, 0.0);
+    k_fill<double><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v64.b, 0.0);
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (!s->elements_set) return fail(OGSX_ESTATE, "ogsx_system_set_elements not called");
+    if (!(dt > 0.0)) return fail(OGSX_EINVAL, "dt must be positive");
+    AsmArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.conn = s->conn;
+    a.color_elems = s->color_elems;
+    a.slot = s->slot;
+    a.x = s->ex;
+    a.y = s->ey;
+    a.z = s->ez;
+    a.mass_coef = s->mass_coef;
+    a.cond = s->cond;
+    a.vel = s->use_vel ? s->vel : nullptr;
+    a.tabN = s->tabN;
+    a.tabdN = s->tabdN;
+    a.tabW = s->tabW;
+    a.u_old = s->uold;
+    a.A = s->A;
+    a.b = s->v64.b;
+    a.cond_stride = s->cond_stride;
+    a.ngp = s->ngp;
+    a.steady = steady;
+    a.tuf = s->tuf;
+    a.mu = s->mu;
+    a.geo_area = s->geo_area;
+    a.adv_coef = s->adv_coef;
+    a.rho_g = s->rho_g;
+    a.dt = dt;
+    a.theta = theta;
+    int pk = prof_begin(s, OGSX_PROF_ASSEMBLY);
+    switch (s->pat->etype) {
+        case OGSX_LINE2: launch_assemble_et<OGSX_LINE2>(s, a); break;
+        case OGSX_QUAD4: launch_assemble_et<OGSX_QUAD4>(s, a); break;
+        case OGSX_HEX8: launch_assemble_et<OGSX_HEX8>(s, a); break;
+        case OGSX_TET4: launch_assemble_et<OGSX_TET4>(s, a); break;
+        default: return fail(OGSX_EINVAL, "unsupported element type");
+    }
+    prof_end(s, pk);
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_add_rhs(ogsx_system* s, int64_t n, const int64_t* idx, const double* val)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (n < 0 || (n > 0 && (!idx || !val))) return fail(OGSX_EINVAL, "bad source-term list");
+    if (n == 0) return OGSX_OK;
+    for (int64_t k = 0; k < n; k++)
+        if (idx[k] < 0 || idx[k] >= s->n) return fail(OGSX_EINVAL, "source-term node out of range");
+    const bool same = (int64_t)s->last_st_idx.size() == n &&
+                      std::equal(idx, idx + n, s->last_st_idx.begin()) &&
+                      std::memcmp(val, s->last_st_val.data(), 8 * n) == 0;
+    if (!same) {
+        // group by node, keeping list order inside a node
+        std::vector<int64_t> ord(n);
+        for (int64_t k = 0; k < n; k++) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return idx[a] < idx[b]; });
+        std::vector<int> node, vptr(1, 0);
+        std::vector<double> v;
+        for (int64_t k = 0; k < n; k++) {
+            const int64_t o = ord[k];
+            if (node.empty() || node.back() != idx[o]) {
+                if (!node.empty()) vptr.push_back((int)v.size());
+                node.push_back((int)idx[o]);
+            }
+            v.push_back(val[o]);
+        }
+        vptr.push_back((int)v.size());
+        int want = (int)std::max<size_t>(v.size(), vptr.size());
+        if (want > s->st_cap) {
+            OK_TRY(dalloc(s, &s->st_node_d, want));
+            OK_TRY(dalloc(s, &s->st_vptr_d, want + 1));
+            OK_TRY(dalloc(s, &s->st_val_d, want));
+            s->st_cap = want;
+        }
+        HIP_TRY(hipMemcpyAsync(s->st_node_d, node.data(), 4 * node.size(), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->st_vptr_d, vptr.data(), 4 * vptr.size(), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->st_val_d, v.data(), 8 * v.size(), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s->st_nu = (int)node.size();
+        s->last_st_idx.assign(idx, idx + n);
+        s->last_st_val.assign(val, val + n);
+    }
+    k_add_rhs<<<blocks_for(s->st_nu, kBlock), kBlock, 0, s->stream>>>(s->st_nu, s->st_node_d,
+                                                                      s->st_vptr_d, s->st_val_d,
+                                                                      s->v64.b);
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx, const double* val, int mode)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (mode != OGSX_DIRICHLET_ROW && mode != OGSX_DIRICHLET_SYMMETRIC)
+        return fail(OGSX_EINVAL, "unknown Dirichlet mode");
+    if (n < 0 || (n > 0 && (!idx || !val))) return fail(OGSX_EINVAL, "bad Dirichlet list");
+    if (n == 0) return OGSX_OK;
+    for (int64_t k = 0; k < n; k++)
+        if (idx[k] < 0 || idx[k] >= s->n) return fail(OGSX_EINVAL, "Dirichlet node out of range");
+    const bool same = (int64_t)s->last_bc_idx.size() == n &&
+                      std::equal(idx, idx + n, s->last_bc_idx.begin()) &&
+                      std::memcmp(val, s->last_bc_val.data(), 8 * n) == 0;
+    if (!same) {
+        // SetKnownX_i applied in list order: a node listed twice keeps the last value
+        std::vector<int64_t> last(s->n, -1);
+        for (int64_t k = 0; k < n; k++) last[idx[k]] = k;
+        std::vector<int> di;
+        std::vector<double> dv;
+        for (int64_t k = 0; k < n; k++)
+            if (last[idx[k]] == k) {
+                di.push_back((int)idx[k]);
+                dv.push_back(val[k]);
+            }
+        OK_TRY(upload_list(s, &s->bc_idx_d, &s->bc_cap, di.data(), (int)di.size()));
+        OK_TRY(upload_list(s, &s->bc_val_d, &s->bc_val_cap, dv.data(), (int)dv.size()));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s->bc_m = (int)di.size();
+        s->last_bc_idx.assign(idx, idx + n);
+        s->last_bc_val.assign(val, val + n);
+    }
+    const int m = s->bc_m;
+    if (mode == OGSX_DIRICHLET_ROW) {
+        k_bc_row<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d, s->ptr,
+                                                                  s->col, s->diagpos, s->A, s->v64.b);
+    } else {
+        k_bc_mark<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d,
+                                                                   s->bcflag, s->bcg, (int8_t)1);
+        k_bc_sym<<<blocks_for(s->n, kBlock), kBlock, 0, s->stream>>>(s->n, s->ptr, s->col, s->diagpos,
+                                                                     s->bcflag, s->bcg, s->A, s->v64.b);
+        k_bc_mark<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d,
+                                                                   s->bcflag, s->bcg, (int8_t)0);
+    }
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* info)
+{
+    if (!s || !cfg) return fail(OGSX_EINVAL, "null argument");
+    if (cfg->method != OGSX_CG && cfg->method != OGSX_BICGSTAB)
+        return fail(OGSX_EINVAL, "method must be OGSX_CG (5) or OGSX_BICGSTAB (2)");
+    int precond = cfg->precond == 100 ? OGSX_PRECOND_JACOBI : cfg->precond;
+    if (precond != OGSX_PRECOND_JACOBI && precond != OGSX_PRECOND_NONE)
+        return fail(OGSX_EINVAL, "unknown preconditioner");
+    if (cfg->max_iter < 0 || !(cfg->tol > 0.0)) return fail(OGSX_EINVAL, "bad tolerance or max_iter");
+    ogsx_solve_info out;
+    std::memset(&out, 0, sizeof(out));
+    const int jac = precond == OGSX_PRECOND_JACOBI;
+    const int nb = blocks_for(s->n, kBlock);
+    if (s->precision == OGSX_FP64) {
+        k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
+        OK_TRY(krylov<double, double>(s, s->A, s->v64, cfg->method, cfg->tol, cfg->max_iter,
+                                      s->bnorm, OGSX_PROF_SPMV64));
+        const KrylovState& st = *s->st_host;
+        out.iterations = st.iter;
+        out.error = st.error;
+        out.bnorm = st.bnorm;
+        out.converged = st.done == 1 ? 1 : st.done == 2 ? 0 : st.done == 4 ? 2 : 3;
+        if (cfg->method == OGSX_BICGSTAB)
+            out.ret = st.done == 2 ? cfg->max_iter + 1 : (st.done == 3 && st.iter == 0) ? 0 : st.iter;
+        else
+            out.ret = st.done == 4 ? 0 : (st.done == 2 ? 0 : 1);
+        if (st.done == 3 && cfg->method == OGSX_BICGSTAB && st.rho1 == 0.0) out.ret = 0;
+        s->bnorm = st.bnorm;
+    } else {
+        // FP32 inner Krylov inside FP64 iterative refinement
+        const int G = s->G;
+        k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
+        k_diag<float><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v32.d, jac);
+        k_cast_f32<<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, s->A32);
+        const double inner_tol = cfg->inner_tol > 0.0 ? cfg->inner_tol : 1e-4;
+        const int max_outer = cfg->max_outer > 0 ? cfg->max_outer : 50;
+        double bnorm_new = 0.0;
+        int total = 0, outer = 0;
+        int converged = 0;
+        double error = 0.0;
+        for (;;) {
+            LPR_DISPATCH(s->lpr, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
+                                     s->n, s->ptr, s->col, s->A, s->v64.x, s->v64.b, s->r64, s->part)));
+            k_reduce_to<<<1, kBlock, 0, s->stream>>>(G, s->part, 2, s->red);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(s->red_host, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            const double bb = s->red_host[0], rr = s->red_host[1];
+            if (outer == 0) {
+                bnorm_new = std::sqrt(bb);
+                // CheckNormRHS (equation_class.cpp:1104)
+                if (s->bnorm > 0.0 && bnorm_new / s->bnorm < cfg->tol) {
+                    error = bnorm_new / s->bnorm;
+                    converged = 2;
+                    break;
+                }
+                if (bnorm_new < DBL_MIN) {
+                    error = 0.0;
+                    converged = 2;
+                    break;
+                }
+            }
+            const double rnorm = std::sqrt(rr);
+            error = rnorm / bnorm_new;
+            if (error < cfg->tol) {
+                converged = 1;
+                break;
+            }
+            if (outer >= max_outer || total >= cfg->max_iter) {
+                converged = 0;
+                break;
+            }
+            k_ir_prepare<<<G, kBlock, 0, s->stream>>>(s->n, s->r64, 1.0 / rnorm, s->v32.b, s->v32.x);
+            const int budget = std::max(1, cfg->max_iter - total);
+            OK_TRY(krylov<float, float>(s, s->A32, s->v32, cfg->method, inner_tol, budget, 0.0,
+                                        OGSX_PROF_SPMV32));
+            const KrylovState& st = *s->st_host;
+            total += std::max(st.iter, 1);
+            if (st.done == 3 && st.iter == 0) {
+                converged = 3;
+                break;
+            }
+            k_ir_correct<<<G, kBlock, 0, s->stream>>>(s->n, s->v64.x, s->v32.x, rnorm);
+            outer++;
+        }
+        out.iterations = total;
+        out.outer = outer;
+        out.error = error;
+        out.bnorm = bnorm_new;
+        out.converged = converged;
+        out.ret = converged == 1 ? total : 0;
+        s->bnorm = bnorm_new;
+    }
+    HIP_TRY(hipGetLastError());
+    if (info) *info = out;
+    return OGSX_OK;
+}
+
+int ogsx_set_vector(ogsx_system* s, int which, const double* host)
+{
+    if (!s || !host) return fail(OGSX_EINVAL, "null argument");
+    double* d = nullptr;
+    OK_TRY(check_vec(s, which, &d));
+    HIP_TRY(hipMemcpyAsync(d, host, 8 * (size_t)s->n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return OGSX_OK;
+}
+
+int ogsx_get_vector(ogsx_system* s, int which, double* host)
+{
+    if (!s || !host) return fail(OGSX_EINVAL, "null argument");
+    double* d = nullptr;
+    OK_TRY(check_vec(s, which, &d));
+    HIP_TRY(hipMemcpyAsync(host, d, 8 * (size_t)s->n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return OGSX_OK;
+}
+
+int ogsx_copy_vector(ogsx_system* s, int dst, int src)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    double *d = nullptr, *q = nullptr;
+    OK_TRY(check_vec(s, dst, &d));
+    OK_TRY(check_vec(s, src, &q));
+    if (d != q) k_copy<double><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, q, d);
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_get_matrix(ogsx_system* s, double* values)
+{
+    if (!s || !values) return fail(OGSX_EINVAL, "null argument");
+    HIP_TRY(hipMemcpyAsync(values, s->A, 8 * (size_t)s->nnz, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return OGSX_OK;
+}
+
+int ogsx_spmv(ogsx_system* s, const double* x, double* y)
+{
+    if (!s || !x || !y) return fail(OGSX_EINVAL, "null argument");
+    // the Krylov work vectors r and s serve as scratch
+    HIP_TRY(hipMemcpyAsync(s->v64.r, x, 8 * (size_t)s->n, hipMemcpyHostToDevice, s->stream));
+    int pk = prof_begin(s, OGSX_PROF_SPMV64);
+    LPR_DISPATCH(s->lpr, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
+                             s->n, s->ptr, s->col, s->A, s->v64.r, s->v64.s)));
+    prof_end(s, pk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(y, s->v64.s, 8 * (size_t)s->n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    OK_TRY(prof_flush(s));
+    return OGSX_OK;
+}
+
+int ogsx_write_bin(ogsx_system* s, const char* path)
+{
+    if (!s || !path) return fail(OGSX_EINVAL, "null argument");
+    const PatternHost& h = *s->pat;
+    std::vector<double> a(h.nnz), b(h.n);
+    OK_TRY(ogsx_get_matrix(s, a.data()));
+    OK_TRY(ogsx_get_vector(s, OGSX_VEC_B, b.data()));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(OGSX_EINVAL, std::string("cannot open ") + path);
+    int64_t n = h.n;
+    bool ok = std::fwrite(&n, 8, 1, f) == 1 &&
+              std::fwrite(h.ptr.data(), 8, h.n + 1, f) == (size_t)(h.n + 1) &&
+              std::fwrite(h.col.data(), 8, h.nnz, f) == (size_t)h.nnz &&
+              std::fwrite(a.data(), 8, h.nnz, f) == (size_t)h.nnz &&
+              std::fwrite(b.data(), 8, h.n, f) == (size_t)h.n;
+    std::fclose(f);
+    return ok ? OGSX_OK : fail(OGSX_EINVAL, "short write");
+}
+
+int ogsx_profile_enable(ogsx_system* s, int on)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (on && s->ev_pool.empty()) {
+        s->ev_pool.resize(1024);
+        for (auto& e : s->ev_pool) HIP_TRY(hipEventCreate(&e));
+    }
+    s->prof = on != 0;
+    for (auto& a : s->prof_acc) a = ProfAcc();
+    return OGSX_OK;
+}
+
+int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms, int64_t* launches,
+                      double* bytes_per_launch)
+{
+    if (!s || kind < 0 || kind > 2) return fail(OGSX_EINVAL, "bad argument");
+    OK_TRY(prof_flush(s));
+    if (total_ms) *total_ms = s->prof_acc[kind].ms;
+    if (launches) *launches = s->prof_acc[kind].launches;
+    if (bytes_per_launch) {
+        const double n = s->n, nnz = (double)s->nnz;
+        if (kind == OGSX_PROF_SPMV64)
+            *bytes_per_launch = nnz * (8 + 4) + (n + 1) * 4 + n * 8 + n * 8;
+        else if (kind == OGSX_PROF_SPMV32)
+            *bytes_per_launch = nnz * (4 + 4) + (n + 1) * 4 + n * 4 + n * 4;
+        else
+            *bytes_per_launch = 0.0;
+    }
+    return OGSX_OK;
+}
+
+int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (lanes_per_row == 0) {
+        s->lpr = choose_lpr(s);
+        return OGSX_OK;
+    }
+    if (lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32 &&
+        lanes_per_row != 64)
+        return fail(OGSX_EINVAL, "lanes_per_row must be 4, 8, 16, 32 or 64");
+    s->lpr = lanes_per_row;
+    return OGSX_OK;
+}
+
+int ogsx_get_spmv_variant(ogsx_system* s, int* lanes_per_row)
+{
+    if (!s || !lanes_per_row) return fail(OGSX_EINVAL, "null argument");
+    *lanes_per_row = s->lpr;
+    return OGSX_OK;
+}
+
+int ogsx_synchronize(ogsx_system* s)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return OGSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,896 @@
+// kernels.hpp -- gfx950 device code of the implicit FEM step.
+//
+// Layout in HBM (one system):
+//   CRS  ptr[n+1], col[nnz] int32; A[nnz] f64 (A32[nnz] f32 in IR mode)
+//   vectors f64[n] (f32[n] for the IR inner solve), Jacobi diagonal d[n]
+//   elements: conn[E*npe] int32, colour-ordered element ids, slot[E*npe*npe]
+//   int32 (CRS position of local entry (i,j)), node coordinates f64 SoA.
+//
+// n-sized kernels split the rows into G contiguous chunks (G a multiple of
+// 8); block b works on chunk (b % 8) * (G / 8) + b / 8, so the chunks that
+// one XCD receives (blocks are dealt to the 8 XCDs round-robin) are adjacent
+// and the x entries a 27-point row gathers stay in that XCD's L2.
+//
+// Reductions are deterministic and atomic-free: every block writes one
+// partial per dot product; the kernel that consumes a dot reduces all G
+// partials in a fixed order (each block redundantly), so results depend only
+// on G and the data, never on scheduling.
+#pragma once
+
+#include <float.h>
+
+#include "ogsx_internal.h"
+
+namespace ogsx {
+
+// ------------------------------------------------------------------------
+// generic helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void chunk_of(int n, int& beg, int& end)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int c = (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
+    beg = (int)((int64_t)n * c / G);
+    end = (int)((int64_t)n * (c + 1) / G);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block-reduces NV values and stores them as partials[k*G + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double* partials)
+{
+    __shared__ double sm[NV][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double w = wave_sum(v[k]);
+        if (lane == 0) sm[k][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = sm[k][0];
+#pragma unroll
+            for (int w = 1; w < kBlock / 64; w++) s += sm[k][w];
+            partials[(size_t)k * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+// Sums G partials per value in a fixed order; every thread gets the result.
+// G is the grid size of the producer, which equals this kernel's grid.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* partials, int G,
+                                                double (&out)[NV])
+{
+    __shared__ double sm[NV][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < G; i += kBlock) s += partials[(size_t)k * G + i];
+        v[k] = wave_sum(s);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sm[k][wave] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = sm[k][0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++) s += sm[k][w];
+        out[k] = s;
+    }
+    __syncthreads();
+}
+
+// Sum over one CRS row by a group of LPR lanes; every lane of the group gets
+// the row sum.  Lanes of one group share `row`, so they enter and leave the
+// row loop together and the width-LPR shuffles stay inside active lanes.
+template <int LPR, typename VT, typename XT, typename ACC>
+__device__ __forceinline__ ACC row_dot(const int* __restrict__ ptr, const int* __restrict__ col,
+                                       const VT* __restrict__ val, const XT* __restrict__ x,
+                                       int row, int lane)
+{
+    const int beg = ptr[row], end = ptr[row + 1];
+    ACC s = 0;
+    for (int k = beg + lane; k < end; k += LPR) s += (ACC)val[k] * (ACC)x[col[k]];
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+    return s;
+}
+
+// Loops the block's chunk of rows with LPR lanes per row and calls
+// epi(row, Ax_row) on the group's lane 0.
+template <int LPR, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_chunk(int n, const int* __restrict__ ptr,
+                                           const int* __restrict__ col, const VT* __restrict__ val,
+                                           const XT* __restrict__ x, Epi&& epi)
+{
+    constexpr int RPB = kBlock / LPR;
+    int beg, end;
+    chunk_of(n, beg, end);
+    const int lane = threadIdx.x % LPR;
+    for (int r = beg + threadIdx.x / LPR; r < end; r += RPB) {
+        ACC s = row_dot<LPR, VT, XT, ACC>(ptr, col, val, x, r, lane);
+        if (lane == 0) epi(r, s);
+    }
+}
+
+template <typename Body>
+__device__ __forceinline__ void vec_chunk(int n, Body&& body)
+{
+    int beg, end;
+    chunk_of(n, beg, end);
+    for (int i = beg + threadIdx.x; i < end; i += kBlock) body(i);
+}
+
+// ------------------------------------------------------------------------
+// element assembly (one thread per element of one colour)
+// ------------------------------------------------------------------------
+struct AsmArgs {
+    const int32_t* conn;
+    const int32_t* color_elems;
+    const int32_t* slot;
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* mass_coef;
+    const double* cond;
+    const double* vel;      // [E][3][ngp] or null
+    const double* tabN;     // [ngp][npe]
+    const double* tabdN;    // [ngp][edim][npe]
+    const double* tabW;     // [ngp][3]
+    const double* u_old;
+    double* A;
+    double* b;
+    int cond_stride;
+    int ngp;
+    int steady;
+    int pad;
+    double tuf, mu, geo_area, adv_coef, rho_g, dt, theta;
+};
+
+template <int ET> struct ETraits;
+template <> struct ETraits<OGSX_LINE2> { static constexpr int NPE = 2, D = 1, NW = 1; };
+template <> struct ETraits<OGSX_QUAD4> { static constexpr int NPE = 4, D = 2, NW = 2; };
+template <> struct ETraits<OGSX_HEX8> { static constexpr int NPE = 8, D = 3, NW = 3; };
+template <> struct ETraits<OGSX_TET4> { static constexpr int NPE = 4, D = 3, NW = 1; };
+
+// Local matrices follow CElement::computeJacobian / ComputeGradShapefct
+// (FEM/fem_ele.cpp:585-729, 1128-1160), CalcMass (FEM/fem_ele_std.cpp:3753),
+// CalcLaplace (:4948), CalcAdvection (:5280), Assemble_Gravity (:5859);
+// the time discretisation follows AssembleParabolicEquation (:7702, LIQUID)
+// and AssembleMixedHyperbolicParabolicEquation (:8504, HEAT).  ISO selects
+// an isotropic (diagonal) tensor: the zero off-diagonal products of the
+// reference's k,l loop add exact zeros and are skipped, and K is formed on
+// its lower triangle and mirrored.
+template <int ET, int PROC, bool ISO>
+__global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
+{
+    constexpr int NPE = ETraits<ET>::NPE, D = ETraits<ET>::D, NW = ETraits<ET>::NW;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int e = a.color_elems[off + t];
+    int nd[NPE];
+    double X[NPE], Y[NPE], Z[NPE];
+#pragma unroll
+    for (int i = 0; i < NPE; i++) {
+        nd[i] = a.conn[(int64_t)e * NPE + i];
+        X[i] = a.x[nd[i]];
+        Y[i] = (D >= 2) ? a.y[nd[i]] : 0.0;
+        Z[i] = (D >= 3) ? a.z[nd[i]] : 0.0;
+    }
+    // CalCoefLaplace: LIQUID tuf*K/mu (fem_ele_std.cpp:2298); HEAT tensor
+    double mat[D * D];
+#pragma unroll
+    for (int i = 0; i < D * D; i++) mat[i] = 0.0;
+    if (ISO) {
+#pragma unroll
+        for (int i = 0; i < D; i++) mat[i * D + i] = a.cond[e];
+    } else {
+#pragma unroll
+        for (int i = 0; i < D * D; i++) mat[i] = a.cond[(int64_t)e * a.cond_stride + i];
+    }
+    if (PROC == OGSX_LIQUID_FLOW) {
+#pragma unroll
+        for (int i = 0; i < D * D; i++) mat[i] = a.tuf * mat[i] / a.mu;
+    }
+    double mass_fac0 = a.mass_coef[e];
+    if (PROC == OGSX_HEAT_TRANSPORT) mass_fac0 /= a.tuf;
+    const bool do_mass = !a.steady;
+    const bool do_adv = (PROC == OGSX_HEAT_TRANSPORT) && a.vel != nullptr;
+    const bool do_grav = (PROC == OGSX_LIQUID_FLOW) && a.rho_g != 0.0;
+
+    double M[NPE * NPE], L[NPE * NPE], Adv[NPE * NPE], G[NPE];
+#pragma unroll
+    for (int i = 0; i < NPE * NPE; i++) {
+        M[i] = 0.0;
+        L[i] = 0.0;
+        Adv[i] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NPE; i++) G[i] = 0.0;
+
+    for (int gp = 0; gp < a.ngp; gp++) {
+        const double* Ng = a.tabN + gp * NPE;
+        const double* dNg = a.tabdN + gp * D * NPE;
+        double J[D * D];
+#pragma unroll
+        for (int i = 0; i < D * D; i++) J[i] = 0.0;
+        double inv[D * D], det;
+        if constexpr (D == 1) {
+            const double dx = X[1] - X[0];
+            J[0] = 0.5 * dx;
+            inv[0] = 2.0 / dx;
+            det = J[0];
+        } else if constexpr (D == 2) {
+#pragma unroll
+            for (int i = 0; i < NPE; i++) {
+                J[0] += X[i] * dNg[i];
+                J[1] += Y[i] * dNg[i];
+                J[2] += X[i] * dNg[NPE + i];
+                J[3] += Y[i] * dNg[NPE + i];
+            }
+            det = J[0] * J[3] - J[1] * J[2];
+            inv[0] = J[3];
+            inv[1] = -J[1];
+            inv[2] = -J[2];
+            inv[3] = J[0];
+#pragma unroll
+            for (int i = 0; i < 4; i++) inv[i] /= det;
+        } else {
+#pragma unroll
+            for (int i = 0; i < NPE; i++) {
+                const double d0 = dNg[i], d1 = dNg[NPE + i], d2 = dNg[2 * NPE + i];
+                J[0] += X[i] * d0;
+                J[1] += Y[i] * d0;
+                J[2] += Z[i] * d0;
+                J[3] += X[i] * d1;
+                J[4] += Y[i] * d1;
+                J[5] += Z[i] * d1;
+                J[6] += X[i] * d2;
+                J[7] += Y[i] * d2;
+                J[8] += Z[i] * d2;
+            }
+            det = J[0] * (J[4] * J[8] - J[7] * J[5]) + J[6] * (J[1] * J[5] - J[4] * J[2]) +
+                  J[3] * (J[2] * J[7] - J[8] * J[1]);
+            inv[0] = J[4] * J[8] - J[7] * J[5];
+            inv[1] = J[2] * J[7] - J[1] * J[8];
+            inv[2] = J[1] * J[5] - J[2] * J[4];
+            inv[3] = J[5] * J[6] - J[8] * J[3];
+            inv[4] = J[0] * J[8] - J[6] * J[2];
+            inv[5] = J[2] * J[3] - J[5] * J[0];
+            inv[6] = J[3] * J[7] - J[6] * J[4];
+            inv[7] = J[1] * J[6] - J[7] * J[0];
+            inv[8] = J[0] * J[4] - J[3] * J[1];
+#pragma unroll
+            for (int i = 0; i < 9; i++) inv[i] /= det;
+        }
+        // global gradients dshp[k][i] = sum_m inv[k][m] dN_local[m][i]
+        double dshp[D * NPE];
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < D; m++) s += inv[k * D + m] * dNg[m * NPE + i];
+                dshp[k * NPE + i] = s;
+            }
+        }
+        double fkt = fabs(det);
+        if (D != 3) fkt *= a.geo_area;
+        fkt = fkt * a.tabW[gp * 3 + 0];
+        if (NW >= 2) fkt = fkt * a.tabW[gp * 3 + 1];
+        if (NW >= 3) fkt = fkt * a.tabW[gp * 3 + 2];
+
+        if (do_mass) {
+            const double mf = mass_fac0 * fkt;
+#pragma unroll
+            for (int i = 0; i < NPE; i++)
+#pragma unroll
+                for (int j = 0; j <= i; j++) M[i * NPE + j] += mf * Ng[i] * Ng[j];
+        }
+        if (ISO) {
+#pragma unroll
+            for (int i = 0; i < NPE; i++)
+#pragma unroll
+                for (int j = 0; j <= i; j++)
+#pragma unroll
+                    for (int k = 0; k < D; k++)
+                        L[i * NPE + j] += fkt * dshp[k * NPE + i] * mat[k * D + k] * dshp[k * NPE + j];
+        } else {
+#pragma unroll
+            for (int i = 0; i < NPE; i++)
+#pragma unroll
+                for (int j = 0; j < NPE; j++)
+#pragma unroll
+                    for (int k = 0; k < D; k++)
+#pragma unroll
+                        for (int l = 0; l < D; l++)
+                            L[i * NPE + j] += fkt * dshp[k * NPE + i] * mat[D * k + l] * dshp[l * NPE + j];
+        }
+        if (do_adv) {
+            double vel[D];
+#pragma unroll
+            for (int k = 0; k < D; k++)
+                vel[k] = a.adv_coef * a.vel[((int64_t)e * 3 + k) * a.ngp + gp];
+#pragma unroll
+            for (int i = 0; i < NPE; i++)
+#pragma unroll
+                for (int j = 0; j < NPE; j++)
+#pragma unroll
+                    for (int k = 0; k < D; k++)
+                        Adv[i * NPE + j] += fkt * Ng[i] * vel[k] * dshp[k * NPE + j];
+        }
+        if (do_grav) {
+            const double f = fkt * a.rho_g;
+#pragma unroll
+            for (int i = 0; i < NPE; i++)
+#pragma unroll
+                for (int k = 0; k < D; k++) G[i] -= f * dshp[k * NPE + i] * mat[D * k + D - 1];
+        }
+    }
+    // mirror the lower triangles (CalcMass :3890-3896)
+#pragma unroll
+    for (int i = 0; i < NPE; i++)
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            M[j * NPE + i] = M[i * NPE + j];
+            if (ISO) L[j * NPE + i] = L[i * NPE + j];
+        }
+
+    const double dt_inverse = 1.0 / a.dt;
+    double u0[NPE];
+#pragma unroll
+    for (int i = 0; i < NPE; i++) u0[i] = a.u_old[nd[i]];
+    double lhs[NPE * NPE], rhs[NPE];
+    if (PROC == OGSX_LIQUID_FLOW) {
+        double relax0 = a.theta;
+        if (relax0 < DBL_MIN) relax0 = 1.0;
+        const double relax1 = 1.0 - relax0;
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+            double val = 0.0;
+#pragma unroll
+            for (int j = 0; j < NPE; j++) {
+                const double m = M[i * NPE + j] * dt_inverse;
+                lhs[i * NPE + j] = m + L[i * NPE + j] * relax0;
+                val += (m - L[i * NPE + j] * relax1) * u0[j];
+            }
+            rhs[i] = val;
+        }
+    } else {
+        const double th = a.theta, fl = -(1.0 - th);
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+            double val = 0.0;
+#pragma unroll
+            for (int j = 0; j < NPE; j++) {
+                const double m = M[i * NPE + j] * dt_inverse;
+                lhs[i * NPE + j] = (m + L[i * NPE + j] * th) + Adv[i * NPE + j] * th;
+                val += ((m + L[i * NPE + j] * fl) + Adv[i * NPE + j] * fl) * u0[j];
+            }
+            rhs[i] = val;
+        }
+    }
+    // atomic-free scatter: elements of one colour own disjoint rows
+    const int32_t* sl = a.slot + (int64_t)e * NPE * NPE;
+#pragma unroll
+    for (int i = 0; i < NPE; i++) {
+        a.b[nd[i]] += rhs[i];
+        if (do_grav) a.b[nd[i]] += G[i];
+#pragma unroll
+        for (int j = 0; j < NPE; j++) a.A[sl[i * NPE + j]] += lhs[i * NPE + j];
+    }
+}
+
+// ------------------------------------------------------------------------
+// sparse matrix-vector product
+// ------------------------------------------------------------------------
+template <int LPR, typename VT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ ptr,
+                                                 const int* __restrict__ col,
+                                                 const VT* __restrict__ val,
+                                                 const XT* __restrict__ x, XT* __restrict__ y)
+{
+    using ACC = XT;
+    spmv_chunk<LPR, VT, XT, ACC>(n, ptr, col, val, x, [&](int r, ACC s) { y[r] = (XT)s; });
+}
+
+// ------------------------------------------------------------------------
+// Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
+// T = vector/arithmetic type, VT = matrix value type.
+// ------------------------------------------------------------------------
+template <typename T>
+struct KVec {
+    T* x;
+    const T* b;
+    const T* d;   // Jacobi diagonal (entries with |a_ii| < DBL_MIN are 1)
+    T* r;
+    T* p;
+    T* s;
+    T* z;
+    T* r0;
+    T* v;
+    T* t;
+    T* ph;
+    T* sh;
+};
+
+// s = A x; r = b - s; z = r/d; p = z; partials (b,b), (r,r), (r,z)
+template <int LPR, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(int n, const int* ptr, const int* col,
+                                                    const VT* val, KVec<T> v, double* part)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.x, [&](int i, T ax) {
+        const T ri = v.b[i] - ax;
+        const T zi = ri / v.d[i];
+        v.r[i] = ri;
+        v.p[i] = zi;
+        acc[0] += (double)v.b[i] * (double)v.b[i];
+        acc[1] += (double)ri * (double)ri;
+        acc[2] += (double)ri * (double)zi;
+    });
+    store_partials<3>(acc, part);
+}
+
+// Single-block: CheckNormRHS (:1104), initial residual test, rr = (r,z).
+__global__ void k_cg_init_final(int G, const double* part, KrylovState* st, double bnorm_prev,
+                                double tol, int max_iter)
+{
+    double s[3];
+    reduce_partials<3>(part, G, s);
+    if (threadIdx.x != 0) return;
+    const double bnorm_new = sqrt(s[0]);
+    st->iter = 0;
+    st->max_iter = max_iter;
+    st->done = 0;
+    if (bnorm_prev > 0.0 && bnorm_new / bnorm_prev < tol) {
+        st->error = bnorm_new / bnorm_prev;
+        st->bnorm = bnorm_new;
+        st->done = 4;
+        return;
+    }
+    st->bnorm = bnorm_new;
+    if (bnorm_new < DBL_MIN) {
+        st->error = 0.0;
+        st->done = 4;
+        return;
+    }
+    st->error = sqrt(s[1]) / bnorm_new;
+    if (st->error < tol) {
+        st->done = 1;
+        return;
+    }
+    st->rr = s[2];
+    st->rr_old = s[2];
+}
+
+// K1: s = A p; partial (p,s)
+template <int LPR, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(int n, const int* ptr, const int* col,
+                                                    const VT* val, KVec<T> v, double* part,
+                                                    const KrylovState* st)
+{
+    if (st->done) return;
+    double acc[1] = {0.0};
+    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.p, [&](int i, T ax) {
+        v.s[i] = ax;
+        acc[0] += (double)v.p[i] * (double)ax;
+    });
+    store_partials<1>(acc, part);
+}
+
+// K2: alpha = rr/(p,s); x += alpha p; r -= alpha s; partials (r,r), (r/d, r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int n, KVec<T> v, const double* part_ps,
+                                                      double* part, KrylovState* st)
+{
+    if (st->done) return;
+    double ps[1];
+    reduce_partials<1>(part_ps, gridDim.x, ps);
+    const double rr = st->rr;
+    const T alpha = (T)(rr / ps[0]);
+    double acc[2] = {0.0, 0.0};
+    vec_chunk(n, [&](int i) {
+        v.x[i] += alpha * v.p[i];
+        const T ri = v.r[i] - alpha * v.s[i];
+        v.r[i] = ri;
+        const T zi = ri / v.d[i];
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)zi * (double)ri;
+    });
+    store_partials<2>(acc, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr_old = rr;
+        st->alpha = rr / ps[0];
+    }
+}
+
+// K3: error test; beta = (z,r)/rr_old; p = r/d + beta p
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_direction(int n, KVec<T> v, const double* part,
+                                                         KrylovState* st, double tol)
+{
+    if (st->done) return;
+    double s[2];
+    reduce_partials<2>(part, gridDim.x, s);
+    const double error = sqrt(s[0]) / st->bnorm;
+    if (error < tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->error = error;
+            st->iter = st->iter + 1;
+            st->done = 1;
+        }
+        return;
+    }
+    const T beta = (T)(s[1] / st->rr_old);
+    vec_chunk(n, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int it = st->iter + 1;
+        st->error = error;
+        st->rr = s[1];
+        st->iter = it;
+        if (it >= st->max_iter) st->done = 2;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Right-preconditioned Jacobi BiCGStab (Linear_EQS::BiCGStab :1287)
+// ------------------------------------------------------------------------
+// s = A x; r0 = b - s; r = r0; v = p = 0; partials (r,r), (r0,r), (b,b)
+// (the first two in the layout k_bi_dir reads)
+template <int LPR, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_init(int n, const int* ptr, const int* col,
+                                                    const VT* val, KVec<T> v, double* part)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.x, [&](int i, T ax) {
+        const T ri = v.b[i] - ax;
+        v.r0[i] = ri;
+        v.r[i] = ri;
+        v.v[i] = 0;
+        v.p[i] = 0;
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)ri * (double)ri;
+        acc[2] += (double)v.b[i] * (double)v.b[i];
+    });
+    store_partials<3>(acc, part);
+}
+
+__global__ void k_bi_init_final(int G, const double* part, KrylovState* st, double bnorm_prev,
+                                double tol, int max_iter)
+{
+    double s[3];
+    reduce_partials<3>(part, G, s);
+    if (threadIdx.x != 0) return;
+    const double bnorm_new = sqrt(s[2]);
+    st->iter = 0;
+    st->max_iter = max_iter;
+    st->done = 0;
+    st->rho0 = 1.0;
+    st->alpha = 1.0;
+    st->omega = 1.0;
+    if (bnorm_prev > 0.0 && bnorm_new / bnorm_prev < tol) {
+        st->error = bnorm_new / bnorm_prev;
+        st->bnorm = bnorm_new;
+        st->done = 4;
+        return;
+    }
+    st->bnorm = bnorm_new;
+    if (bnorm_new < DBL_MIN) {
+        st->error = 0.0;
+        st->done = 4;
+        return;
+    }
+    st->error = sqrt(s[0]) / bnorm_new;
+    if (st->error < tol) st->done = 1;
+}
+
+// B1: test of the previous iterate's r; rho1 = (r0,r); p update; p_h = p/d
+// part_rr: partials [(r,r), (r0,r)] from k_bi_init or k_bi_update.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_dir(int n, KVec<T> v, const double* part_rr,
+                                                   KrylovState* st, double tol)
+{
+    if (st->done) return;
+    double s[2];
+    reduce_partials<2>(part_rr, gridDim.x, s);
+    const int it = st->iter;
+    if (it > 0) {
+        const double err = sqrt(s[0]) / st->bnorm;
+        if (err < tol || fabs(st->omega) < DBL_MIN) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                st->error = err;
+                st->done = (err < tol) ? 1 : 3;
+            }
+            return;
+        }
+        if (it >= st->max_iter) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                st->error = err;
+                st->done = 2;
+            }
+            return;
+        }
+    }
+    const double rho1 = s[1];
+    if (fabs(rho1) < DBL_MIN) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
+        return;
+    }
+    if (it == 0) {
+        vec_chunk(n, [&](int i) {
+            const T pi = v.r[i];
+            v.p[i] = pi;
+            v.ph[i] = pi / v.d[i];
+        });
+    } else {
+        const T beta = (T)((rho1 / st->rho0) * (st->alpha / st->omega));
+        const T omega = (T)st->omega;
+        vec_chunk(n, [&](int i) {
+            const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
+            v.p[i] = pi;
+            v.ph[i] = pi / v.d[i];
+        });
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rho1 = rho1;
+}
+
+// B2: v = A p_h; partial (r0,v); counts the iteration
+template <int LPR, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(int n, const int* ptr, const int* col,
+                                                      const VT* val, KVec<T> v, double* part,
+                                                      KrylovState* st)
+{
+    if (st->done) return;
+    double acc[1] = {0.0};
+    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.ph, [&](int i, T ax) {
+        v.v[i] = ax;
+        acc[0] += (double)v.r0[i] * (double)ax;
+    });
+    store_partials<1>(acc, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+}
+
+// B3: alpha = rho1/(r0,v); s = r - alpha v; s_h = s/d; partial (s,s)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_s(int n, KVec<T> v, const double* part_r0v,
+                                                 double* part, KrylovState* st)
+{
+    if (st->done) return;
+    double s[1];
+    reduce_partials<1>(part_r0v, gridDim.x, s);
+    const double rho1 = st->rho1;
+    const double alpha_d = rho1 / s[0];
+    const T alpha = (T)alpha_d;
+    double acc[1] = {0.0};
+    vec_chunk(n, [&](int i) {
+        const T si = v.r[i] - alpha * v.v[i];
+        v.s[i] = si;
+        v.sh[i] = si / v.d[i];
+        acc[0] += (double)si * (double)si;
+    });
+    store_partials<1>(acc, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->alpha = alpha_d;
+        st->rho0 = rho1;
+    }
+}
+
+// B4: test on s (then x += alpha p_h and stop); else t = A s_h;
+// partials (t,t), (t,s)
+template <int LPR, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(int n, const int* ptr, const int* col,
+                                                      const VT* val, KVec<T> v,
+                                                      const double* part_ss, double* part,
+                                                      KrylovState* st, double tol)
+{
+    if (st->done) return;
+    double ss[1];
+    reduce_partials<1>(part_ss, gridDim.x, ss);
+    const double err = sqrt(ss[0]) / st->bnorm;
+    if (err < tol) {
+        const T alpha = (T)st->alpha;
+        vec_chunk(n, [&](int i) { v.x[i] += alpha * v.ph[i]; });
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->error = err;
+            st->done = 1;
+        }
+        return;
+    }
+    double acc[2] = {0.0, 0.0};
+    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.sh, [&](int i, T ax) {
+        v.t[i] = ax;
+        acc[0] += (double)ax * (double)ax;
+        acc[1] += (double)ax * (double)v.s[i];
+    });
+    store_partials<2>(acc, part);
+}
+
+// B5: omega = (t,s)/(t,t); x += alpha p_h + omega s_h; r = s - omega t;
+// partials (r,r), (r0,r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_update(int n, KVec<T> v, const double* part_t,
+                                                      double* part, KrylovState* st)
+{
+    if (st->done) return;
+    double s[2];
+    reduce_partials<2>(part_t, gridDim.x, s);
+    const double omega_d = (s[0] > DBL_MIN) ? s[1] / s[0] : 1.0;
+    const T omega = (T)omega_d;
+    const T alpha = (T)st->alpha;
+    double acc[2] = {0.0, 0.0};
+    vec_chunk(n, [&](int i) {
+        v.x[i] += alpha * v.ph[i] + omega * v.sh[i];
+        const T ri = v.s[i] - omega * v.t[i];
+        v.r[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)v.r0[i] * (double)ri;
+    });
+    store_partials<2>(acc, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->omega = omega_d;
+}
+
+// ------------------------------------------------------------------------
+// FP32-inside-FP64 iterative refinement helpers
+// ------------------------------------------------------------------------
+// r = b - A x in FP64; partials (b,b), (r,r)
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void k_ir_residual(int n, const int* ptr, const int* col,
+                                                        const double* val, const double* x,
+                                                        const double* b, double* r, double* part)
+{
+    double acc[2] = {0.0, 0.0};
+    spmv_chunk<LPR, double, double, double>(n, ptr, col, val, x, [&](int i, double ax) {
+        const double ri = b[i] - ax;
+        r[i] = ri;
+        acc[0] += b[i] * b[i];
+        acc[1] += ri * ri;
+    });
+    store_partials<2>(acc, part);
+}
+
+__global__ void k_reduce_to(int G, const double* part, int nv, double* out)
+{
+    double s[2];
+    reduce_partials<2>(part, G, s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nv && k < 2; k++) out[k] = s[k];
+}
+
+// r32 = r64 * scale; x32 = 0
+__global__ __launch_bounds__(kBlock) void k_ir_prepare(int n, const double* r, double scale,
+                                                       float* r32, float* x32)
+{
+    vec_chunk(n, [&](int i) {
+        r32[i] = (float)(r[i] * scale);
+        x32[i] = 0.0f;
+    });
+}
+
+// x += scale * d32
+__global__ __launch_bounds__(kBlock) void k_ir_correct(int n, double* x, const float* d32,
+                                                       double scale)
+{
+    vec_chunk(n, [&](int i) { x[i] += scale * (double)d32[i]; });
+}
+
+// ------------------------------------------------------------------------
+// small utility kernels
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t n, T* p, T v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_f32(int64_t n, const double* a, float* b)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        b[i] = (float)a[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const T* a, T* b)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        b[i] = a[i];
+}
+
+// Jacobi diagonal: Precond_Jacobi's a_ii with |a_ii| < DBL_MIN -> 1
+// (FEM/matrix_class.cpp:1786-1821); no preconditioner -> 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag(int n, const int* diagpos, const double* A,
+                                                 T* d, int jacobi)
+{
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        double v = (jacobi && diagpos[i] >= 0) ? A[diagpos[i]] : 1.0;
+        if (fabs(v) < DBL_MIN) v = 1.0;
+        d[i] = (T)v;
+    }
+}
+
+// Point source terms: one thread per distinct node, values in list order.
+__global__ __launch_bounds__(kBlock) void k_add_rhs(int nu, const int* node, const int* vptr,
+                                                    const double* val, double* b)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nu) return;
+    double acc = b[node[k]];
+    for (int j = vptr[k]; j < vptr[k + 1]; j++) acc += val[j];
+    b[node[k]] = acc;
+}
+
+// Dirichlet flags: flag[i] = 1 and g[i] for every listed node
+__global__ __launch_bounds__(kBlock) void k_bc_mark(int m, const int* idx, const double* val,
+                                                    int8_t* flag, double* g, int8_t on)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m) return;
+    flag[idx[k]] = on;
+    if (on) g[idx[k]] = val[k];
+}
+
+// Row-only elimination (CSparseMatrix::Diagonize CRS branch,
+// matrix_class.cpp:1663-1690,1772): one thread per Dirichlet node.
+__global__ __launch_bounds__(kBlock) void k_bc_row(int m, const int* idx, const double* val,
+                                                   const int* ptr, const int* col,
+                                                   const int* diagpos, double* A, double* b)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m) return;
+    const int id = idx[k];
+    const int dp = diagpos[id];
+    const double vdiag = dp >= 0 ? A[dp] : 0.0;
+    for (int j = ptr[id]; j < ptr[id + 1]; j++)
+        if (col[j] != id) A[j] = 0.0;
+    b[id] = vdiag * val[k];
+}
+
+// Symmetric elimination: Dirichlet rows as above; every other row lifts
+// b_i -= a_ik g_k and zeroes a_ik for Dirichlet columns k (column order).
+__global__ __launch_bounds__(kBlock) void k_bc_sym(int n, const int* ptr, const int* col,
+                                                   const int* diagpos, const int8_t* flag,
+                                                   const double* g, double* A, double* b)
+{
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        if (flag[i]) {
+            const int dp = diagpos[i];
+            const double vdiag = dp >= 0 ? A[dp] : 0.0;
+            for (int j = ptr[i]; j < ptr[i + 1]; j++)
+                if (col[j] != i) A[j] = 0.0;
+            b[i] = vdiag * g[i];
+        } else {
+            double bi = b[i];
+            bool touched = false;
+            for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+                const int c = col[j];
+                if (flag[c]) {
+                    bi -= A[j] * g[c];
+                    A[j] = 0.0;
+                    touched = true;
+                }
+            }
+            if (touched) b[i] = bi;
+        }
+    }
+}
+
+}  // namespace ogsx

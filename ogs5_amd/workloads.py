@@ -1,0 +1,202 @@
+"""Synthetic scenarios of the benchmark configurations and the time-step
+driver that calls the C ABI in the order CRFProcess::Execute does
+(FEM/rf_pcs.cpp:4695-4853): Initialize -> x0 = u(t_n) -> GlobalAssembly ->
+IncorporateSourceTerms -> IncorporateBoundaryConditions -> Solver -> copy
+back u(t_n+1) = x.
+
+Scenarios (all seeded, all synthetic):
+  quad_flow   2-D transient confined groundwater flow on bilinear quads
+              (200 x 200 over a 1000 m square is the CPU-runnable config)
+  hex_flow    3-D liquid flow on trilinear hexes (128^3 is the headline
+              config): one steady solve, then transient steps from the IC
+  tet_heat    advection-diffusion heat transport on 6-tet-split cubes
+  bmskel      the reference's own 1-element HEAT_TRANSPORT deck
+              (tests/data/bmskel)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib as L
+from .fields import divergence_free_velocity, gauss_point_coords, lognormal_conductivity
+from .mesh import Mesh, hex_mesh, line_mesh, quad_mesh, tet_mesh
+
+
+@dataclass
+class Step:
+    dt: float
+    theta: float
+    steady: bool = False
+    reset_to_ic: bool = False  # start from the initial condition again
+
+
+@dataclass
+class Scenario:
+    name: str
+    mesh: Mesh
+    process: int
+    mass_coef: np.ndarray
+    cond: np.ndarray
+    cond_stride: int
+    bc_idx: np.ndarray
+    bc_val: np.ndarray
+    st_idx: np.ndarray
+    st_val: np.ndarray
+    u0: np.ndarray
+    steps: List[Step]
+    tol: float = 1e-10
+    max_iter: int = 100000
+    n_gauss: int = 3
+    tuf: float = 1.0
+    mu: float = 1.0
+    geo_area: float = 1.0
+    adv_coef: float = 0.0
+    vel: Optional[np.ndarray] = None
+    rho_g: float = 0.0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return self.mesh.n_nodes
+
+    def element_kwargs(self) -> dict:
+        m = self.mesh
+        return dict(process=self.process, dim=m.dim, x=m.x, y=m.y if m.dim >= 2 else None,
+                    z=m.z if m.dim >= 3 else None, mass_coef=self.mass_coef, cond=self.cond,
+                    cond_stride=self.cond_stride, n_gauss=self.n_gauss, tuf=self.tuf, mu=self.mu,
+                    geo_area=self.geo_area, adv_coef=self.adv_coef, vel=self.vel,
+                    rho_g=self.rho_g)
+
+
+def quad_flow(nx: int = 200, ny: int = 200, *, length: float = 1000.0, seed: int = 1,
+              mean_ln_k: float = np.log(1e-4), sigma_ln_k: float = 1.0, corr_len: float = 50.0,
+              storage: float = 1e-4, head_left: float = 10.0, head_right: float = 0.0,
+              sink: float = -0.01, dt: float = 3600.0, n_steps: int = 10) -> Scenario:
+    """2-D transient confined flow (the 200x200 / 1000 m configuration by default).
+
+    Dirichlet head on the left and right edges, one point sink at the centre
+    node, implicit Euler (theta = 1), x0 = previous time level.
+    """
+    m = quad_mesh(nx, ny, length, length * ny / nx)
+    K = lognormal_conductivity(m, mean_ln_k, sigma_ln_k, corr_len, seed)
+    left = m.nodes_where(0, 0.0)
+    right = m.nodes_where(0, length)
+    bc_idx = np.concatenate([left, right])
+    bc_val = np.concatenate([np.full(left.size, head_left), np.full(right.size, head_right)])
+    centre = m.node_index(nx // 2, ny // 2)
+    return Scenario(f"quad_flow_{nx}x{ny}", m, L.LIQUID_FLOW, np.full(m.n_elem, storage), K, 1,
+                    bc_idx.astype(np.int64), bc_val, np.array([centre], np.int64),
+                    np.array([sink]), np.zeros(m.n_nodes), [Step(dt, 1.0)] * n_steps,
+                    meta=dict(seed=seed, sigma_ln_k=sigma_ln_k))
+
+
+def hex_flow(n: int = 128, *, seed: int = 2, mean_ln_k: float = np.log(1e-4),
+             sigma_ln_k: float = 2.0, corr_cells: float = 8.0, storage: float = 1e-4,
+             dh: float = 1.0, dt: float = 100.0, n_transient: int = 5,
+             steady: bool = True) -> Scenario:
+    """3-D liquid flow on n^3 unit hexes: head dh on x = 0, 0 on x = n.
+
+    Steps: one steady solve (zero storage) from h = 0, then n_transient
+    implicit-Euler steps that restart from the initial condition h = 0.
+    """
+    m = hex_mesh(n, n, n)
+    K = lognormal_conductivity(m, mean_ln_k, sigma_ln_k, corr_cells, seed)
+    left = m.nodes_where(0, 0.0)
+    right = m.nodes_where(0, float(n))
+    bc_idx = np.concatenate([left, right]).astype(np.int64)
+    bc_val = np.concatenate([np.full(left.size, dh), np.zeros(right.size)])
+    steps = []
+    if steady:
+        steps.append(Step(dt, 1.0, steady=True))
+    for k in range(n_transient):
+        steps.append(Step(dt, 1.0, reset_to_ic=(k == 0 and steady)))
+    return Scenario(f"hex_flow_{n}^3", m, L.LIQUID_FLOW, np.full(m.n_elem, storage), K, 1,
+                    bc_idx, bc_val, np.zeros(0, np.int64), np.zeros(0), np.zeros(m.n_nodes),
+                    steps, meta=dict(seed=seed, sigma_ln_k=sigma_ln_k))
+
+
+def tet_heat(n: int = 96, *, seed: int = 3, conductivity: float = 2.0, rho_c: float = 2.5e6,
+             fluid_rho_c: float = 4.2e6, peclet_range=(1.0, 20.0), t0: float = 10.0,
+             t_inlet: float = 60.0, dt: float = 1.0e5, n_steps: int = 20,
+             theta: float = 0.5) -> Scenario:
+    """Advection-diffusion of heat on n^3 cubes split into 6 tets each.
+
+    Gauss-point velocity v = curl(0,0,psi) (divergence-free), scaled so the
+    grid Peclet c_f rho_f |v| h / (2 lambda) spans peclet_range; inlet face
+    x = 0 held at t_inlet; Crank-Nicolson (theta = 0.5).
+    """
+    m = tet_mesh(n, n, n)
+    h = m.h[0]
+    pe_lo, pe_hi = peclet_range
+    # |v| in [u(1-a), u(1+a)] with (1+a)/(1-a) = pe_hi/pe_lo
+    a = (pe_hi / pe_lo - 1.0) / (pe_hi / pe_lo + 1.0)
+    u_mean = pe_lo * 2.0 * conductivity / (fluid_rho_c * h) / (1.0 - a)
+    gp = gauss_point_coords(m, 3)  # [E, 5, 3]
+    v = divergence_free_velocity(gp, u_mean, a * 0.999, float(n) * h, seed)  # [E, 5, 3]
+    vel = np.ascontiguousarray(np.transpose(v, (0, 2, 1)))  # [E][3][ngp]
+    inlet = m.nodes_where(0, 0.0)
+    speed = np.linalg.norm(v, axis=-1)
+    pe = fluid_rho_c * speed * h / (2.0 * conductivity)
+    return Scenario(f"tet_heat_{n}^3x6", m, L.HEAT_TRANSPORT, np.full(m.n_elem, rho_c),
+                    np.full(m.n_elem, conductivity), 1, inlet, np.full(inlet.size, t_inlet),
+                    np.zeros(0, np.int64), np.zeros(0), np.full(m.n_nodes, t0),
+                    [Step(dt, theta)] * n_steps, adv_coef=fluid_rho_c, vel=vel,
+                    meta=dict(seed=seed, peclet_min=float(pe.min()), peclet_max=float(pe.max())))
+
+
+def bmskel() -> Scenario:
+    """The reference's tests/data/bmskel deck: one line element of length 1,
+    HEAT_TRANSPORT, T0 = 273, Dirichlet 274 at POINT0 (node 0), one step of
+    dt = 1 with theta = 0.5 and BiCGStab + Jacobi to 1e-12.  Solid density
+    2000, capacity 900, conductivity 5 (a.msp)."""
+    m = line_mesh(1, 1.0)
+    return Scenario("bmskel", m, L.HEAT_TRANSPORT, np.array([2000.0 * 900.0]), np.array([5.0]), 1,
+                    np.array([0], np.int64), np.array([274.0]), np.zeros(0, np.int64),
+                    np.zeros(0), np.full(2, 273.0), [Step(1.0, 0.5)], tol=1e-12, max_iter=1000)
+
+
+# ---------------------------------------------------------------------------
+# GPU driver
+# ---------------------------------------------------------------------------
+def make_system(sc: Scenario, precision: int = L.FP64):
+    pat = L.Pattern(sc.n, sc.mesh.etype, sc.mesh.conn)
+    sys_ = L.System(pat, precision)
+    sys_.set_elements(**sc.element_kwargs())
+    return pat, sys_
+
+
+def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SYMMETRIC,
+            precision: int = L.FP64, inner_tol: float = 1e-4, system=None, keep_steps=False,
+            tol: Optional[float] = None) -> dict:
+    """Runs every step of the scenario on the GPU; returns the final x, the
+    per-step solve info and (keep_steps) every step's solution."""
+    if system is None:
+        pat, s = make_system(sc, precision)
+    else:
+        pat, s = system
+    s.set_vector(L.VEC_UOLD, sc.u0)
+    s.set_vector(L.VEC_X, sc.u0)
+    infos, xs = [], []
+    for st in sc.steps:
+        if st.reset_to_ic:
+            s.set_vector(L.VEC_UOLD, sc.u0)
+            s.set_vector(L.VEC_X, sc.u0)
+        s.initialize()
+        s.assemble(st.dt, st.theta, st.steady)
+        if sc.st_idx.size:
+            s.add_rhs(sc.st_idx, sc.st_val)
+        s.set_dirichlet(sc.bc_idx, sc.bc_val, dirichlet)
+        info = s.solve(method=method, tol=sc.tol if tol is None else tol,
+                       max_iter=sc.max_iter, inner_tol=inner_tol)
+        infos.append(info)
+        s.copy_vector(L.VEC_UOLD, L.VEC_X)
+        if keep_steps:
+            xs.append(s.get_vector(L.VEC_X))
+    x = s.get_vector(L.VEC_X)
+    out = dict(x=x, info=infos, system=(pat, s))
+    if keep_steps:
+        out["steps"] = xs
+    return out

@@ -1,0 +1,955 @@
+/*
+ * oracle.c -- CPU restatement of OGS-5's implicit FEM step (LIQUID_FLOW and
+ * HEAT_TRANSPORT assembly + Linear_EQS CG/BiCGStab with Jacobi).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the serial
+ * CPU baseline.  The product path (ogs5_amd/) never links or calls it.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   Gauss tables, shape functions       compared value-for-value with the
+ *                                       reference's own mathlib.cpp compiled
+ *                                       into oracle/_ref (when buildable);
+ *                                       partition of unity, sum dN = 0,
+ *                                       weight sums.
+ *   CRS pattern                         closed-form nnz of 9/27/15-point
+ *                                       stencils; brute-force adjacency on
+ *                                       random small meshes; sortedness.
+ *   Jacobian / grad N / |J|             sum of |J| w = element area/volume,
+ *                                       affine elements vs analytic gradients.
+ *   CalcMass / CalcLaplace              closed forms for square quads
+ *                                       (K[2/3,-1/6,-1/3,-1/6], M A/36[4,2,1,2])
+ *                                       and cubes; K*1 = 0; symmetry; tets vs
+ *                                       exact P1 integrals.
+ *   CalcAdvection                       A*1 = 0; exact P1 integral on tets.
+ *   Parabolic / mixed-hyperbolic LHS+RHS time-discretisation identities.
+ *   Dense Matrix::multi                 tests/Matrix/testMatrix.cpp values,
+ *                                       and _ref's Matrix::multi.
+ *   Dirichlet (row-only / symmetric)    structure + same solution.
+ *   CG / BiCGStab / Jacobi              dense LU solve (numpy) on small
+ *                                       systems; reference's bmskel golden
+ *                                       (T(POINT0) = 274 after one step).
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------
+ * a4: Gauss points and weights.  The 15-digit literals are the reference's
+ * own quadrature table (FEM/mathlib.cpp:1112-1215); they must be identical
+ * for the assembled entries to agree to the last bits.
+ * ---------------------------------------------------------------------- */
+double orc_gauss_pkt(long grd, long pkt)
+{
+    static const double p2[2] = {0.577350269189626, -0.577350269189626};
+    static const double p3[3] = {0.774596669241483, 0.0, -0.774596669241483};
+    static const double p4[4] = {0.861136311594053, 0.339981043584856,
+                                 -0.339981043584856, -0.861136311594053};
+    if (grd == 1) return 0.0;
+    if (grd == 2 && pkt >= 0 && pkt < 2) return p2[pkt];
+    if (grd == 3 && pkt >= 0 && pkt < 3) return p3[pkt];
+    if (grd == 4 && pkt >= 0 && pkt < 4) return p4[pkt];
+    return 0.0;
+}
+
+double orc_gauss_fkt(long grd, long pkt)
+{
+    static const double w3[3] = {0.555555555555556, 0.888888888888889,
+                                 0.555555555555556};
+    static const double w4[4] = {0.347854845137454, 0.652145154862546,
+                                 0.652145154862546, 0.347854845137454};
+    if (grd == 1) return 2.0;
+    if (grd == 2 && pkt >= 0 && pkt < 2) return 1.0;
+    if (grd == 3 && pkt >= 0 && pkt < 3) return w3[pkt];
+    if (grd == 4 && pkt >= 0 && pkt < 4) return w4[pkt];
+    return 0.0;
+}
+
+/* 5-point tetrahedron rule with a negative centre weight
+ * (FEM/mathlib.cpp:1285-1322). */
+void orc_sample_tet5(int gp, double* sp)
+{
+    static const double t[5][4] = {
+        {0.25, 0.25, 0.25, -0.133333333333333},
+        {0.166666666666667, 0.166666666666667, 0.166666666666667, 0.075},
+        {0.5, 0.166666666666667, 0.166666666666667, 0.075},
+        {0.166666666666667, 0.5, 0.166666666666667, 0.075},
+        {0.166666666666667, 0.166666666666667, 0.5, 0.075}};
+    if (gp < 0 || gp > 4) return;
+    for (int k = 0; k < 4; k++) sp[k] = t[gp][k];
+}
+
+int orc_npe(int etype)
+{
+    switch (etype) {
+        case ORC_LINE2: return 2;
+        case ORC_QUAD4: return 4;
+        case ORC_HEX8: return 8;
+        case ORC_TET4: return 4;
+    }
+    return 0;
+}
+
+int orc_edim(int etype)
+{
+    switch (etype) {
+        case ORC_LINE2: return 1;
+        case ORC_QUAD4: return 2;
+        case ORC_HEX8: return 3;
+        case ORC_TET4: return 3;
+    }
+    return 0;
+}
+
+/* CElement::SetIntegrationPointNumber (FEM/fem_ele.cpp:362). */
+int orc_ngp(int etype, int n_gauss)
+{
+    switch (etype) {
+        case ORC_LINE2: return n_gauss;
+        case ORC_QUAD4: return n_gauss * n_gauss;
+        case ORC_HEX8: return n_gauss * n_gauss * n_gauss;
+        case ORC_TET4: return 5;
+    }
+    return 0;
+}
+
+/* SetGaussPoint (fem_ele.cpp:813) gives the unit coordinates; GetGaussData
+ * (fem_ele.cpp:885) multiplies |J| by the weights left to right, so the
+ * weights are returned as a list (wfac[0..nw-1]) rather than a product. */
+void orc_gauss_point(int etype, int n_gauss, int gp, double unit[3],
+                     double wfac[3], int* nw)
+{
+    unit[0] = unit[1] = unit[2] = 0.0;
+    switch (etype) {
+        case ORC_LINE2:
+            unit[0] = orc_gauss_pkt(n_gauss, gp);
+            wfac[0] = orc_gauss_fkt(n_gauss, gp);
+            *nw = 1;
+            return;
+        case ORC_QUAD4: {
+            int r = gp / n_gauss, s = gp % n_gauss;
+            unit[0] = orc_gauss_pkt(n_gauss, r);
+            unit[1] = orc_gauss_pkt(n_gauss, s);
+            wfac[0] = orc_gauss_fkt(n_gauss, r);
+            wfac[1] = orc_gauss_fkt(n_gauss, s);
+            *nw = 2;
+            return;
+        }
+        case ORC_HEX8: {
+            int r = gp / (n_gauss * n_gauss);
+            int s = gp % (n_gauss * n_gauss);
+            int t = s % n_gauss;
+            s /= n_gauss;
+            unit[0] = orc_gauss_pkt(n_gauss, r);
+            unit[1] = orc_gauss_pkt(n_gauss, s);
+            unit[2] = orc_gauss_pkt(n_gauss, t);
+            wfac[0] = orc_gauss_fkt(n_gauss, r);
+            wfac[1] = orc_gauss_fkt(n_gauss, s);
+            wfac[2] = orc_gauss_fkt(n_gauss, t);
+            *nw = 3;
+            return;
+        }
+        case ORC_TET4: {
+            double sp[4];
+            orc_sample_tet5(gp, sp);
+            unit[0] = sp[0];
+            unit[1] = sp[1];
+            unit[2] = sp[2];
+            wfac[0] = sp[3];
+            *nw = 1;
+            return;
+        }
+    }
+    *nw = 0;
+}
+
+/* Shape functions (mathlib.cpp:1661 line, :1880 quad, :2079 tet, :2224 hex).
+ * Node 0 of the quad sits at (+1,+1); of the hex at (+1,+1,+1). */
+void orc_shape(int etype, const double* u, double* N)
+{
+    switch (etype) {
+        case ORC_LINE2:
+            N[0] = 1.0 - u[0];
+            N[1] = 1.0 + u[0];
+            for (int i = 0; i < 2; i++) N[i] *= 0.5;
+            return;
+        case ORC_QUAD4:
+            N[0] = (1.0 + u[0]) * (1.0 + u[1]);
+            N[1] = (1.0 - u[0]) * (1.0 + u[1]);
+            N[2] = (1.0 - u[0]) * (1.0 - u[1]);
+            N[3] = (1.0 + u[0]) * (1.0 - u[1]);
+            for (int i = 0; i < 4; i++) N[i] *= 0.25;
+            return;
+        case ORC_TET4:
+            N[0] = 1. - u[0] - u[1] - u[2];
+            N[1] = u[0];
+            N[2] = u[1];
+            N[3] = u[2];
+            return;
+        case ORC_HEX8:
+            N[0] = (1.0 + u[0]) * (1.0 + u[1]) * (1.0 + u[2]);
+            N[1] = (1.0 - u[0]) * (1.0 + u[1]) * (1.0 + u[2]);
+            N[2] = (1.0 - u[0]) * (1.0 - u[1]) * (1.0 + u[2]);
+            N[3] = (1.0 + u[0]) * (1.0 - u[1]) * (1.0 + u[2]);
+            N[4] = (1.0 + u[0]) * (1.0 + u[1]) * (1.0 - u[2]);
+            N[5] = (1.0 - u[0]) * (1.0 + u[1]) * (1.0 - u[2]);
+            N[6] = (1.0 - u[0]) * (1.0 - u[1]) * (1.0 - u[2]);
+            N[7] = (1.0 + u[0]) * (1.0 - u[1]) * (1.0 - u[2]);
+            for (int i = 0; i < 8; i++) N[i] *= 0.125;
+            return;
+    }
+}
+
+/* Local gradients, layout dN[k*npe + i] = dN_i/dxi_k
+ * (mathlib.cpp:1694 line, :1908 quad, :2103 tet, :2264 hex). */
+void orc_grad_shape(int etype, const double* u, double* dN)
+{
+    switch (etype) {
+        case ORC_LINE2:
+            dN[0] = -0.5;
+            dN[1] = 0.5;
+            return;
+        case ORC_QUAD4:
+            dN[0] = +(1.0 + u[1]);
+            dN[1] = -(1.0 + u[1]);
+            dN[2] = -(1.0 - u[1]);
+            dN[3] = +(1.0 - u[1]);
+            dN[4] = +(1.0 + u[0]);
+            dN[5] = +(1.0 - u[0]);
+            dN[6] = -(1.0 - u[0]);
+            dN[7] = -(1.0 + u[0]);
+            for (int i = 0; i < 8; i++) dN[i] *= 0.25;
+            return;
+        case ORC_TET4: {
+            static const double g[12] = {-1.0, 1.0, 0.0, 0.0, -1.0, 0.0,
+                                         1.0,  0.0, -1.0, 0.0, 0.0, 1.0};
+            for (int i = 0; i < 12; i++) dN[i] = g[i];
+            return;
+        }
+        case ORC_HEX8: {
+            const double r = u[0], s = u[1], t = u[2];
+            dN[0] = +(1.0 + s) * (1.0 + t);
+            dN[1] = -(1.0 + s) * (1.0 + t);
+            dN[2] = -(1.0 - s) * (1.0 + t);
+            dN[3] = +(1.0 - s) * (1.0 + t);
+            dN[4] = +(1.0 + s) * (1.0 - t);
+            dN[5] = -(1.0 + s) * (1.0 - t);
+            dN[6] = -(1.0 - s) * (1.0 - t);
+            dN[7] = +(1.0 - s) * (1.0 - t);
+            dN[8] = +(1.0 + r) * (1.0 + t);
+            dN[9] = +(1.0 - r) * (1.0 + t);
+            dN[10] = -(1.0 - r) * (1.0 + t);
+            dN[11] = -(1.0 + r) * (1.0 + t);
+            dN[12] = +(1.0 + r) * (1.0 - t);
+            dN[13] = +(1.0 - r) * (1.0 - t);
+            dN[14] = -(1.0 - r) * (1.0 - t);
+            dN[15] = -(1.0 + r) * (1.0 - t);
+            dN[16] = +(1.0 + r) * (1.0 + s);
+            dN[17] = +(1.0 - r) * (1.0 + s);
+            dN[18] = +(1.0 - r) * (1.0 - s);
+            dN[19] = +(1.0 + r) * (1.0 - s);
+            dN[20] = -(1.0 + r) * (1.0 + s);
+            dN[21] = -(1.0 - r) * (1.0 + s);
+            dN[22] = -(1.0 - r) * (1.0 - s);
+            dN[23] = -(1.0 + r) * (1.0 - s);
+            for (int i = 0; i < 24; i++) dN[i] *= 0.125;
+            return;
+        }
+    }
+}
+
+/* ShapeFunctionPool::computeQuadratures (FEM/ShapeFunctionPool.cpp:109). */
+void orc_shape_pool(int etype, int n_gauss, double* N, double* dN)
+{
+    const int ngp = orc_ngp(etype, n_gauss), npe = orc_npe(etype);
+    const int ed = orc_edim(etype);
+    for (int gp = 0; gp < ngp; gp++) {
+        double unit[3], w[3];
+        int nw;
+        orc_gauss_point(etype, n_gauss, gp, unit, w, &nw);
+        orc_shape(etype, unit, &N[gp * npe]);
+        orc_grad_shape(etype, unit, &dN[gp * npe * ed]);
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * a1/a2: CRS pattern.  CFEMesh::ConnectedNodes (msh_mesh.cpp:2931) collects
+ * the distinct nodes of every element touching a node and sorts them; the
+ * CRS table (matrix_class.cpp:743-777) copies them row by row and records
+ * the diagonal position.  Linear elements keep every column (< rows).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int64_t* ptr;  /* node -> element incidence */
+    int64_t* ele;
+} orc_incidence;
+
+static void build_incidence(int64_t n_nodes, int64_t n_elem, int npe,
+                            const int64_t* conn, orc_incidence* inc)
+{
+    inc->ptr = (int64_t*)calloc((size_t)n_nodes + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < n_elem; e++)
+        for (int a = 0; a < npe; a++) inc->ptr[conn[e * npe + a] + 1]++;
+    for (int64_t i = 0; i < n_nodes; i++) inc->ptr[i + 1] += inc->ptr[i];
+    inc->ele = (int64_t*)malloc(sizeof(int64_t) * (size_t)(inc->ptr[n_nodes] + 1));
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_nodes + 1));
+    memcpy(fill, inc->ptr, sizeof(int64_t) * (size_t)n_nodes);
+    for (int64_t e = 0; e < n_elem; e++)
+        for (int a = 0; a < npe; a++) {
+            int64_t nd = conn[e * npe + a];
+            /* An element listing a node twice is recorded once. */
+            int dup = 0;
+            for (int b = 0; b < a; b++)
+                if (conn[e * npe + b] == nd) dup = 1;
+            if (!dup) inc->ele[fill[nd]++] = e;
+        }
+    /* shrink per-node lists that lost duplicates */
+    int64_t w = 0;
+    for (int64_t i = 0; i < n_nodes; i++) {
+        int64_t s = inc->ptr[i], t = fill[i];
+        inc->ptr[i] = w;
+        for (int64_t k = s; k < t; k++) inc->ele[w++] = inc->ele[k];
+    }
+    inc->ptr[n_nodes] = w;
+    free(fill);
+}
+
+/* Distinct connected nodes of node i in ascending order; returns count. */
+static int connected_nodes(const orc_incidence* inc, int npe,
+                           const int64_t* conn, int64_t i, int64_t* buf,
+                           int cap)
+{
+    int cnt = 0;
+    for (int64_t k = inc->ptr[i]; k < inc->ptr[i + 1]; k++) {
+        const int64_t e = inc->ele[k];
+        for (int l = 0; l < npe; l++) {
+            const int64_t nd = conn[e * npe + l];
+            int exist = 0;
+            for (int m = 0; m < cnt; m++)
+                if (buf[m] == nd) {
+                    exist = 1;
+                    break;
+                }
+            if (!exist && cnt < cap) buf[cnt++] = nd;
+        }
+    }
+    /* the reference's exchange sort (msh_mesh.cpp:2965-2978) */
+    for (int k = 0; k < cnt; k++)
+        for (int l = k; l < cnt; l++)
+            if (buf[l] < buf[k]) {
+                int64_t t = buf[k];
+                buf[k] = buf[l];
+                buf[l] = t;
+            }
+    return cnt;
+}
+
+#define ORC_MAX_ADJ 4096
+
+int64_t orc_pattern_ptr(int64_t n_nodes, int64_t n_elem, int npe,
+                        const int64_t* conn, int64_t* row_ptr)
+{
+    orc_incidence inc;
+    build_incidence(n_nodes, n_elem, npe, conn, &inc);
+    int64_t* buf = (int64_t*)malloc(sizeof(int64_t) * ORC_MAX_ADJ);
+    row_ptr[0] = 0;
+    for (int64_t i = 0; i < n_nodes; i++) {
+        int c = connected_nodes(&inc, npe, conn, i, buf, ORC_MAX_ADJ);
+        int kept = 0;
+        for (int k = 0; k < c; k++)
+            if (buf[k] < n_nodes) kept++;
+        row_ptr[i + 1] = row_ptr[i] + kept;
+    }
+    free(buf);
+    free(inc.ptr);
+    free(inc.ele);
+    return row_ptr[n_nodes];
+}
+
+void orc_pattern_fill(int64_t n_nodes, int64_t n_elem, int npe,
+                      const int64_t* conn, const int64_t* row_ptr,
+                      int64_t* col, int64_t* diag)
+{
+    orc_incidence inc;
+    build_incidence(n_nodes, n_elem, npe, conn, &inc);
+    int64_t* buf = (int64_t*)malloc(sizeof(int64_t) * ORC_MAX_ADJ);
+    for (int64_t i = 0; i < n_nodes; i++) {
+        int c = connected_nodes(&inc, npe, conn, i, buf, ORC_MAX_ADJ);
+        int64_t w = row_ptr[i];
+        diag[i] = -1;
+        for (int k = 0; k < c; k++) {
+            if (buf[k] >= n_nodes) continue;
+            if (buf[k] == i) diag[i] = w;
+            col[w++] = buf[k];
+        }
+    }
+    free(buf);
+    free(inc.ptr);
+    free(inc.ele);
+}
+
+/* ------------------------------------------------------------------------
+ * a5: element geometry.  computeJacobian (fem_ele.cpp:585-729) and
+ * ComputeGradShapefct (fem_ele.cpp:1128-1160) for linear elements whose
+ * dimension equals the problem dimension.
+ * ---------------------------------------------------------------------- */
+static int element_geometry(const orc_problem* p, int64_t e, int gp,
+                            const double* X, const double* Y, const double* Z,
+                            double* dshp /* edim*npe */, double* det_out)
+{
+    const int et = p->etype, npe = orc_npe(et), ed = orc_edim(et);
+    double unit[3], w[3], dloc[24], J[9], inv[9];
+    int nw;
+    orc_gauss_point(et, p->n_gauss, gp, unit, w, &nw);
+    orc_grad_shape(et, unit, dloc);
+    for (int i = 0; i < ed * ed; i++) J[i] = 0.0;
+    double det = 0.0;
+    if (ed == 1) {
+        const double dx = X[1] - X[0];
+        J[0] = 0.5 * dx;
+        inv[0] = 2.0 / dx;
+        det = J[0];
+    } else if (ed == 2) {
+        for (int i = 0, j = npe; i < npe; i++, j++) {
+            J[0] += X[i] * dloc[i];
+            J[1] += Y[i] * dloc[i];
+            J[2] += X[i] * dloc[j];
+            J[3] += Y[i] * dloc[j];
+        }
+        det = J[0] * J[3] - J[1] * J[2];
+        if (fabs(det) < DBL_EPSILON) return -1; /* MKleinsteZahl: abort() */
+        inv[0] = J[3];
+        inv[1] = -J[1];
+        inv[2] = -J[2];
+        inv[3] = J[0];
+        for (int i = 0; i < 4; i++) inv[i] /= det;
+    } else {
+        for (int i = 0; i < npe; i++) {
+            const int j = i + npe, k = i + 2 * npe;
+            J[0] += X[i] * dloc[i];
+            J[1] += Y[i] * dloc[i];
+            J[2] += Z[i] * dloc[i];
+            J[3] += X[i] * dloc[j];
+            J[4] += Y[i] * dloc[j];
+            J[5] += Z[i] * dloc[j];
+            J[6] += X[i] * dloc[k];
+            J[7] += Y[i] * dloc[k];
+            J[8] += Z[i] * dloc[k];
+        }
+        det = J[0] * (J[4] * J[8] - J[7] * J[5]) +
+              J[6] * (J[1] * J[5] - J[4] * J[2]) +
+              J[3] * (J[2] * J[7] - J[8] * J[1]);
+        if (fabs(det) < DBL_EPSILON) return -1;
+        inv[0] = J[4] * J[8] - J[7] * J[5];
+        inv[1] = J[2] * J[7] - J[1] * J[8];
+        inv[2] = J[1] * J[5] - J[2] * J[4];
+        inv[3] = J[5] * J[6] - J[8] * J[3];
+        inv[4] = J[0] * J[8] - J[6] * J[2];
+        inv[5] = J[2] * J[3] - J[5] * J[0];
+        inv[6] = J[3] * J[7] - J[6] * J[4];
+        inv[7] = J[1] * J[6] - J[7] * J[0];
+        inv[8] = J[0] * J[4] - J[3] * J[1];
+        for (int i = 0; i < 9; i++) inv[i] /= det;
+    }
+    for (int i = 0; i < npe; i++) {
+        double var[3];
+        for (int j = 0; j < ed; j++) {
+            var[j] = dloc[j * npe + i];
+            dshp[j * npe + i] = 0.0;
+        }
+        for (int j = 0, jk = 0; j < ed; j++)
+            for (int k = 0; k < ed; k++, jk++)
+                dshp[j * npe + i] += inv[jk] * var[k];
+    }
+    /* |det| (fem_ele.cpp:728); geo_area for ele_dim != 3 (fem_ele_std.cpp:950) */
+    double d = fabs(det);
+    if (ed != 3) d *= p->geo_area;
+    /* GetGaussData: |J| * w_r * w_s * w_t, left to right */
+    for (int k = 0; k < nw; k++) d = d * w[k];
+    *det_out = d;
+    (void)e;
+    return 0;
+}
+
+static void load_coords(const orc_problem* p, int64_t e, double* X, double* Y,
+                        double* Z)
+{
+    const int npe = orc_npe(p->etype);
+    for (int a = 0; a < npe; a++) {
+        const int64_t nd = p->conn[e * npe + a];
+        X[a] = p->x[nd];
+        Y[a] = p->y ? p->y[nd] : 0.0;
+        Z[a] = p->z ? p->z[nd] : 0.0;
+    }
+}
+
+/* mat[dim*dim] = CalCoefLaplace (fem_ele_std.cpp:2202): LIQUID
+ * tuf*K/mu (:2298-2301); HEAT the conductivity tensor (:2538-2543). */
+static void laplace_tensor(const orc_problem* p, int64_t e, double* mat)
+{
+    const int dim = p->dim;
+    double tensor[9];
+    for (int i = 0; i < dim * dim; i++) tensor[i] = 0.0;
+    if (p->cond_stride == 1) {
+        for (int i = 0; i < dim; i++) tensor[i * dim + i] = p->cond[e];
+    } else {
+        for (int i = 0; i < dim * dim; i++)
+            tensor[i] = p->cond[e * p->cond_stride + i];
+    }
+    if (p->process == ORC_LIQUID_FLOW) {
+        for (int i = 0; i < dim * dim; i++)
+            mat[i] = p->tuf * tensor[i] / p->mu * 1.0 * 1.0;
+    } else {
+        for (int i = 0; i < dim * dim; i++) mat[i] = tensor[i];
+    }
+}
+
+void orc_element_matrices(const orc_problem* p, int64_t e, double* mass,
+                          double* laplace, double* advection, double* grav)
+{
+    const int et = p->etype, npe = orc_npe(et), ed = orc_edim(et);
+    const int ngp = orc_ngp(et, p->n_gauss), dim = p->dim;
+    double X[8], Y[8], Z[8], dshp[24], N[8], mat[9];
+    load_coords(p, e, X, Y, Z);
+    laplace_tensor(p, e, mat);
+    if (mass)
+        for (int i = 0; i < npe * npe; i++) mass[i] = 0.0;
+    if (laplace)
+        for (int i = 0; i < npe * npe; i++) laplace[i] = 0.0;
+    if (advection)
+        for (int i = 0; i < npe * npe; i++) advection[i] = 0.0;
+    if (grav)
+        for (int i = 0; i < npe; i++) grav[i] = 0.0;
+    /* steady: HEAT skips CalcMass for tim_type STEADY (fem_ele_std.cpp:8543);
+     * a steady LIQUID solve is the same operator with zero storage. */
+    const int do_mass = mass && !p->steady;
+    for (int gp = 0; gp < ngp; gp++) {
+        double fkt, unit[3], w[3];
+        int nw;
+        if (element_geometry(p, e, gp, X, Y, Z, dshp, &fkt) != 0) {
+            fprintf(stderr, "oracle: Jacobian determinant == 0 in element %lld\n",
+                    (long long)e);
+            abort();
+        }
+        (void)ed;
+        orc_gauss_point(et, p->n_gauss, gp, unit, w, &nw);
+        orc_shape(et, unit, N);
+        /* CalcMass (:3753): lower triangle, then mirrored */
+        if (do_mass) {
+            double mat_fac = p->mass_coef[e];
+            if (p->process == ORC_HEAT_TRANSPORT) mat_fac /= p->tuf;
+            mat_fac *= fkt;
+            mat_fac *= 1.0; /* ElementVolumeMultiplyer */
+            for (int i = 0; i < npe; i++)
+                for (int j = 0; j <= i; j++)
+                    mass[i * npe + j] += mat_fac * N[i] * N[j];
+        }
+        /* CalcLaplace (:4948) */
+        if (laplace)
+            for (int i = 0; i < npe; i++)
+                for (int j = 0; j < npe; j++)
+                    for (int k = 0; k < dim; k++)
+                        for (int l = 0; l < dim; l++)
+                            laplace[i * npe + j] += fkt * dshp[k * npe + i] *
+                                                    mat[dim * k + l] *
+                                                    dshp[l * npe + j];
+        /* CalcAdvection (:5280) with CalCoefAdvection c_f*rho_f (:3633) */
+        if (advection && p->process == ORC_HEAT_TRANSPORT && p->vel) {
+            double vel[3];
+            for (int k = 0; k < 3; k++)
+                vel[k] = p->adv_coef * p->vel[(e * 3 + k) * ngp + gp];
+            for (int i = 0; i < npe; i++)
+                for (int j = 0; j < npe; j++)
+                    for (int k = 0; k < dim; k++)
+                        advection[i * npe + j] +=
+                            fkt * N[i] * vel[k] * dshp[k * npe + j];
+        }
+        /* Assemble_Gravity (:5859): NodalVal[i] -= fkt*rho_g*dN_ki*mat[k,dim-1] */
+        if (grav && p->process == ORC_LIQUID_FLOW && p->rho_g != 0.0) {
+            const double f = fkt * p->rho_g;
+            for (int i = 0; i < npe; i++)
+                for (int k = 0; k < dim; k++)
+                    grav[i] -= f * dshp[k * npe + i] * mat[dim * k + dim - 1];
+        }
+    }
+    if (do_mass)
+        for (int i = 0; i < npe; i++)
+            for (int j = 0; j < npe; j++)
+                if (i > j) mass[j * npe + i] = mass[i * npe + j];
+}
+
+void orc_dense_multi(const double* A, int nrows, int ncols, const double* vec,
+                     double* vec_result, double fac)
+{
+    for (int i = 0; i < nrows; i++) {
+        double val = 0.;
+        const double* row = &A[i * ncols];
+        for (int j = 0; j < ncols; j++) val += row[j] * vec[j];
+        vec_result[i] += fac * val;
+    }
+}
+
+/* LHS, RHS and the gravity vector of one element; gravity is added to b
+ * after the RHS, as Assemble_Gravity runs after the time discretisation. */
+static void element_system_full(const orc_problem* p, int64_t e, double* lhs,
+                                double* rhs, double* G)
+{
+    const int npe = orc_npe(p->etype), nn = npe * npe;
+    double M[64], L[64], A[64], aux[64], aux1[64], u0[8];
+    orc_element_matrices(p, e, M, L, A, G);
+    const double dt_inverse = 1.0 / p->dt;
+    for (int a = 0; a < npe; a++) u0[a] = p->u_old[p->conn[e * npe + a]];
+    if (p->process == ORC_LIQUID_FLOW) {
+        /* AssembleParabolicEquation (:7702): relax0 = theta, relax1 = 1-theta */
+        double relax0 = p->theta;
+        if (relax0 < DBL_MIN) relax0 = 1.0;
+        const double relax1 = 1.0 - relax0;
+        for (int i = 0; i < nn; i++) lhs[i] = M[i] * dt_inverse;
+        for (int i = 0; i < nn; i++) aux[i] = L[i] * relax0;
+        for (int i = 0; i < nn; i++) lhs[i] += aux[i];
+        for (int i = 0; i < nn; i++) aux1[i] = M[i] * dt_inverse;
+        for (int i = 0; i < nn; i++) aux[i] = L[i] * relax1;
+        for (int i = 0; i < nn; i++) aux1[i] -= aux[i];
+        for (int a = 0; a < npe; a++) rhs[a] = 0.0;
+        orc_dense_multi(aux1, npe, npe, u0, rhs, 1.0);
+    } else {
+        /* AssembleMixedHyperbolicParabolicEquation (:8504), S = C = 0 */
+        const double th = p->theta;
+        for (int i = 0; i < nn; i++) lhs[i] = M[i] * dt_inverse;
+        for (int i = 0; i < nn; i++) aux[i] = L[i] * th;
+        for (int i = 0; i < nn; i++) lhs[i] += aux[i];
+        for (int i = 0; i < nn; i++) aux[i] = A[i] * th;
+        for (int i = 0; i < nn; i++) lhs[i] += aux[i];
+        const double fl = -(1.0 - th);
+        for (int i = 0; i < nn; i++) aux1[i] = M[i] * dt_inverse;
+        for (int i = 0; i < nn; i++) aux[i] = L[i] * fl;
+        for (int i = 0; i < nn; i++) aux1[i] += aux[i];
+        for (int i = 0; i < nn; i++) aux[i] = A[i] * fl;
+        for (int i = 0; i < nn; i++) aux1[i] += aux[i];
+        for (int a = 0; a < npe; a++) rhs[a] = 0.0;
+        orc_dense_multi(aux1, npe, npe, u0, rhs, 1.0);
+    }
+}
+
+void orc_element_system(const orc_problem* p, int64_t e, double* lhs,
+                        double* rhs)
+{
+    double G[8];
+    const int npe = orc_npe(p->etype);
+    element_system_full(p, e, lhs, rhs, G);
+    for (int a = 0; a < npe; a++) rhs[a] += G[a];
+}
+
+/* ------------------------------------------------------------------------
+ * Linear_EQS
+ * ---------------------------------------------------------------------- */
+void orc_eqs_init(orc_eqs* q, int64_t n, int64_t nnz, const int64_t* ptr,
+                  const int64_t* col, const int64_t* diag, double* a,
+                  double* b, double* x)
+{
+    memset(q, 0, sizeof(*q));
+    q->n = n;
+    q->nnz = nnz;
+    q->ptr = ptr;
+    q->col = col;
+    q->diag = diag;
+    q->a = a;
+    q->b = b;
+    q->x = x;
+    q->bnorm = 1.0;   /* equation_class.cpp:135 */
+    q->error = 1.0e10;
+    q->method = 2;
+    q->precond = 1;
+    q->tol = 1e-10;
+    q->max_iter = 1000;
+}
+
+void orc_eqs_config(orc_eqs* q, int method, int precond, double tol,
+                    int max_iter)
+{
+    q->method = method;
+    q->precond = (precond == 100) ? 1 : precond; /* :285-291 ILU -> Jacobi */
+    q->tol = tol;
+    q->max_iter = max_iter;
+}
+
+void orc_eqs_initialize(orc_eqs* q)
+{
+    for (int64_t k = 0; k < q->nnz; k++) q->a[k] = 0.;
+    for (int64_t i = 0; i < q->n; i++) q->b[i] = 0.;
+    q->error = 1.0e10;
+}
+
+/* binarySearch (FEM/mathlib.cpp:3102).  The reference passes the row bounds
+ * [ptr[i], ptr[i+1]] as an inclusive range; for the (row, column) pairs that
+ * element assembly queries the column is always inside the row, so a
+ * half-open search returns the same slot. */
+static int64_t bsearch_col(const int64_t* col, int64_t key, int64_t beg,
+                           int64_t end)
+{
+    int64_t lo = beg, hi = end;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (col[mid] == key) return mid;
+        if (col[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return -1;
+}
+
+void orc_assemble(orc_eqs* q, const orc_problem* p)
+{
+    const int npe = orc_npe(p->etype);
+    double lhs[64], rhs[8], G[8];
+    const int grav = p->process == ORC_LIQUID_FLOW && p->rho_g != 0.0;
+    for (int64_t e = 0; e < p->n_elem; e++) {
+        element_system_full(p, e, lhs, rhs, G);
+        const int64_t* en = &p->conn[e * npe];
+        for (int i = 0; i < npe; i++) q->b[en[i]] += rhs[i];
+        if (grav)
+            for (int i = 0; i < npe; i++) q->b[en[i]] += G[i];
+        for (int i = 0; i < npe; i++) {
+            const int64_t kk = en[i];
+            for (int j = 0; j < npe; j++) {
+                const int64_t k =
+                    bsearch_col(q->col, en[j], q->ptr[kk], q->ptr[kk + 1]);
+                if (k >= 0) q->a[k] += lhs[i * npe + j];
+            }
+        }
+    }
+}
+
+void orc_add_rhs(orc_eqs* q, int64_t n, const int64_t* idx, const double* val)
+{
+    for (int64_t k = 0; k < n; k++) q->b[idx[k]] += val[k];
+}
+
+/* CSparseMatrix::Diagonize, CRS branch (matrix_class.cpp:1663-1690, 1772). */
+void orc_set_known_x(orc_eqs* q, int64_t id, double g)
+{
+    const int64_t j = q->diag[id];
+    const double vdiag = (j >= 0) ? q->a[j] : 0.0;
+    for (int64_t k = q->ptr[id]; k < q->ptr[id + 1]; k++)
+        if (q->col[k] != id) q->a[k] = 0.;
+    q->b[id] = vdiag * g;
+}
+
+void orc_set_known_x_sym(orc_eqs* q, int64_t id, double g)
+{
+    const int64_t j = q->diag[id];
+    const double vdiag = (j >= 0) ? q->a[j] : 0.0;
+    for (int64_t k = q->ptr[id]; k < q->ptr[id + 1]; k++)
+        if (q->col[k] != id) q->a[k] = 0.;
+    for (int64_t i = 0; i < q->n; i++) {
+        if (i == id) continue;
+        const int64_t k = bsearch_col(q->col, id, q->ptr[i], q->ptr[i + 1]);
+        if (k == -1) continue;
+        q->b[i] -= q->a[k] * g;
+        q->a[k] = 0.;
+    }
+    q->b[id] = vdiag * g;
+}
+
+void orc_spmv(const orc_eqs* q, const double* x, double* y)
+{
+    for (int64_t i = 0; i < q->n; i++) y[i] = 0.0;
+    for (int64_t i = 0; i < q->n; i++)
+        for (int64_t k = q->ptr[i]; k < q->ptr[i + 1]; k++)
+            y[i] += q->a[k] * x[q->col[k]];
+}
+
+void orc_precond_jacobi(const orc_eqs* q, const double* r, double* z)
+{
+    for (int64_t i = 0; i < q->n; i++) {
+        double d = (q->diag[i] >= 0) ? q->a[q->diag[i]] : 0.0;
+        if (fabs(d) < DBL_MIN) d = 1.0;
+        z[i] = r[i] / d;
+    }
+}
+
+static void precond(const orc_eqs* q, const double* r, double* z)
+{
+    if (q->precond == 1)
+        orc_precond_jacobi(q, r, z);
+    else
+        memcpy(z, r, sizeof(double) * (size_t)q->n);
+}
+
+double orc_dot(int64_t n, const double* a, const double* b)
+{
+    double v = 0.;
+    for (int64_t i = 0; i < n; i++) v += a[i] * b[i];
+    return v;
+}
+
+static double norm2(int64_t n, const double* a) { return sqrt(orc_dot(n, a, a)); }
+
+/* CheckNormRHS (equation_class.cpp:1104). */
+static int check_norm_rhs(orc_eqs* q, double normb_new)
+{
+    if (q->bnorm > 0.0)
+        if ((normb_new / q->bnorm) < q->tol) {
+            q->error = normb_new / q->bnorm;
+            q->bnorm = normb_new;
+            return 1;
+        }
+    q->bnorm = normb_new;
+    if (q->bnorm < DBL_MIN) {
+        q->error = 0.;
+        return 1;
+    }
+    return 0;
+}
+
+int orc_cg(orc_eqs* q)
+{
+    const int64_t n = q->n;
+    double* p = (double*)malloc(sizeof(double) * (size_t)n);
+    double* r = (double*)malloc(sizeof(double) * (size_t)n);
+    double* s = (double*)malloc(sizeof(double) * (size_t)n);
+    int ret;
+    q->iter = 0;
+    if (check_norm_rhs(q, norm2(n, q->b))) {
+        ret = 0;
+        goto done;
+    }
+    orc_spmv(q, q->x, s);
+    for (int64_t i = 0; i < n; i++) r[i] = q->b[i] - s[i];
+    precond(q, r, s);
+    for (int64_t i = 0; i < n; i++) p[i] = s[i];
+    if ((q->error = norm2(n, r) / q->bnorm) < q->tol) {
+        ret = 1;
+        goto done;
+    }
+    double rr = orc_dot(n, r, s);
+    for (q->iter = 1; q->iter <= q->max_iter; ++q->iter) {
+        orc_spmv(q, p, s);
+        const double alpha = rr / orc_dot(n, p, s);
+        for (int64_t i = 0; i < n; i++) {
+            q->x[i] += alpha * p[i];
+            r[i] -= alpha * s[i];
+        }
+        if ((q->error = norm2(n, r) / q->bnorm) < q->tol) {
+            ret = q->iter <= q->max_iter;
+            goto done;
+        }
+        precond(q, r, s);
+        const double rrM1 = rr;
+        rr = orc_dot(n, s, r);
+        const double beta = rr / rrM1;
+        for (int64_t i = 0; i < n; i++) p[i] = s[i] + beta * p[i];
+    }
+    ret = q->iter <= q->max_iter;
+done:
+    free(p);
+    free(r);
+    free(s);
+    return ret;
+}
+
+int orc_bicgstab(orc_eqs* q)
+{
+    const int64_t n = q->n;
+    double* buf = (double*)malloc(sizeof(double) * (size_t)n * 8);
+    double *r0 = buf, *r = buf + n, *s = buf + 2 * n, *s_h = buf + 3 * n;
+    double *t = buf + 4 * n, *v = buf + 5 * n, *p = buf + 6 * n;
+    double* p_h = buf + 7 * n;
+    double rho_0 = 1.0, rho_1, alpha = 1.0, beta, omega = 1.0, tt, norm_r;
+    int ret;
+    q->iter = 0;
+    if (check_norm_rhs(q, norm2(n, q->b))) {
+        ret = 0;
+        goto done;
+    }
+    orc_spmv(q, q->x, s);
+    for (int64_t i = 0; i < n; i++) r0[i] = q->b[i] - s[i];
+    for (int64_t i = 0; i < n; i++) {
+        r[i] = r0[i];
+        v[i] = 0.;
+        p[i] = 0.;
+    }
+    if ((q->error = norm2(n, r) / q->bnorm) < q->tol) {
+        ret = 0;
+        goto done;
+    }
+    for (q->iter = 1; q->iter <= q->max_iter; q->iter++) {
+        rho_1 = orc_dot(n, r0, r);
+        if (fabs(rho_1) < DBL_MIN) {
+            ret = 0;
+            goto done;
+        }
+        if (q->iter == 1)
+            for (int64_t i = 0; i < n; i++) p[i] = r[i];
+        else {
+            beta = (rho_1 / rho_0) * (alpha / omega);
+            for (int64_t i = 0; i < n; i++)
+                p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        }
+        precond(q, p, p_h);
+        orc_spmv(q, p_h, v);
+        alpha = rho_1 / orc_dot(n, r0, v);
+        for (int64_t i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
+        if ((q->error = norm2(n, s) / q->bnorm) < q->tol) {
+            for (int64_t i = 0; i < n; i++) q->x[i] += alpha * p_h[i];
+            ret = q->iter;
+            goto done;
+        }
+        precond(q, s, s_h);
+        orc_spmv(q, s_h, t);
+        tt = orc_dot(n, t, t);
+        if (tt > DBL_MIN)
+            omega = orc_dot(n, t, s) / tt;
+        else
+            omega = 1.0;
+        for (int64_t i = 0; i < n; i++) {
+            q->x[i] += alpha * p_h[i] + omega * s_h[i];
+            r[i] = s[i] - omega * t[i];
+        }
+        rho_0 = rho_1;
+        norm_r = norm2(n, r);
+        if ((q->error = norm_r / q->bnorm) < q->tol) {
+            ret = q->iter;
+            goto done;
+        }
+        if (fabs(omega) < DBL_MIN) {
+            q->error = norm_r / q->bnorm;
+            ret = q->iter;
+            goto done;
+        }
+    }
+    ret = q->iter;
+done:
+    free(buf);
+    return ret;
+}
+
+int orc_solver(orc_eqs* q)
+{
+    q->iter = 0;
+    switch (q->method) {
+        case 2:
+            /* iter = BiCGStab(); return iter; (:868-870) */
+            q->iter = orc_bicgstab(q);
+            return q->iter;
+        case 5:
+            return orc_cg(q);
+    }
+    return -1;
+}
+
+int orc_write_bin(const orc_eqs* q, const char* path)
+{
+    FILE* f = fopen(path, "wb");
+    if (!f) return -1;
+    int64_t n = q->n;
+    fwrite(&n, sizeof(int64_t), 1, f);
+    fwrite(q->ptr, sizeof(int64_t), (size_t)n + 1, f);
+    fwrite(q->col, sizeof(int64_t), (size_t)q->ptr[n], f);
+    fwrite(q->a, sizeof(double), (size_t)q->ptr[n], f);
+    fwrite(q->b, sizeof(double), (size_t)n, f);
+    fclose(f);
+    return 0;
+}

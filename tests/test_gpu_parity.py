@@ -1,0 +1,255 @@
+"""HIP path vs the oracle, through the C ABI, on the same seeded inputs.
+
+Bars: CRS index arrays bit-exact; assembled FP64 entries within 1e-12 of
+the oracle's, relative to the largest entry of their row (entries that cancel
+to ~0 in the global sum are compared on the row's scale); right-hand sides
+within 1e-12 relative to max|b|; solutions within 1e-10 relative L2 of a
+tight-tolerance oracle solve.
+"""
+import numpy as np
+import pytest
+
+import ogs5_amd as ox
+from ogs5_amd import workloads as W
+from ogs5_amd.fields import lognormal_conductivity
+from ogs5_amd.mesh import hex_mesh
+from oracle import driver
+
+pytestmark = pytest.mark.gpu
+
+ENTRY_TOL = 1e-12
+SOL_TOL = 1e-10
+
+
+def _scenarios():
+    out = {
+        "quad": W.quad_flow(20, 16, seed=5, n_steps=3),
+        "hex": W.hex_flow(6, seed=6, n_transient=2),
+        "tet": W.tet_heat(5, seed=7, n_steps=3),
+        "bmskel": W.bmskel(),
+    }
+    # full anisotropic tensor, distorted hexes, gravity on
+    sc = W.hex_flow(5, seed=8, n_transient=2, steady=False)
+    rng = np.random.default_rng(8)
+    m = sc.mesh
+    m.x = m.x + 0.15 * rng.uniform(-1, 1, m.n_nodes)
+    m.y = m.y + 0.15 * rng.uniform(-1, 1, m.n_nodes)
+    T = np.array([[2.0, 0.3, 0.1], [0.3, 1.0, -0.2], [0.1, -0.2, 0.5]]) * 1e-4
+    sc.cond = np.tile(T.ravel(), (m.n_elem, 1)) * sc.cond[:, None] / 1e-4
+    sc.cond_stride = 9
+    sc.rho_g = 1000.0 * 9.81
+    out["hex_aniso_grav"] = sc
+    return out
+
+
+SCEN = _scenarios()
+
+
+def _row_scaled_close(a, a_ref, ptr, tol):
+    n = ptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    rowmax = np.zeros(n)
+    np.maximum.at(rowmax, rows, np.abs(a_ref))
+    err = np.abs(a - a_ref) / np.maximum(rowmax[rows], 1e-300)
+    return err.max()
+
+
+def _assemble_both(sc, bc_mode, step=0):
+    """Assembles one step (with ST and, unless bc_mode is None, Dirichlet) on
+    the GPU and in the oracle; returns (A_gpu, b_gpu, A_ref, b_ref, ptr)."""
+    from oracle import ffi
+    pat, s = W.make_system(sc)
+    st = sc.steps[step]
+    s.set_vector(ox.VEC_UOLD, sc.u0)
+    s.initialize()
+    s.assemble(st.dt, st.theta, st.steady)
+    if sc.st_idx.size:
+        s.add_rhs(sc.st_idx, sc.st_val)
+    if bc_mode is not None:
+        s.set_dirichlet(sc.bc_idx, sc.bc_val, bc_mode)
+    A, b = s.get_matrix(), s.get_vector(ox.VEC_B)
+    ptr, col, diag = ffi.pattern(sc.n, sc.mesh.conn)
+    eq = ffi.OracleEqs(ptr, col, diag)
+    prob = ffi.OracleProblem(sc.mesh, process=sc.process, mass_coef=sc.mass_coef, cond=sc.cond,
+                             cond_stride=sc.cond_stride, n_gauss=sc.n_gauss, tuf=sc.tuf,
+                             mu=sc.mu, geo_area=sc.geo_area, adv_coef=sc.adv_coef, vel=sc.vel,
+                             rho_g=sc.rho_g, dt=st.dt, theta=st.theta, steady=st.steady,
+                             u_old=sc.u0)
+    eq.initialize()
+    eq.assemble(prob)
+    if sc.st_idx.size:
+        eq.add_rhs(sc.st_idx, sc.st_val)
+    if bc_mode is not None:
+        eq.set_known_x(sc.bc_idx, sc.bc_val, symmetric=(bc_mode == ox.DIRICHLET_SYMMETRIC))
+    p_exp = pat.export()
+    assert np.array_equal(p_exp[0], ptr) and np.array_equal(p_exp[1], col)
+    return A, b, eq.a.copy(), eq.b.copy(), ptr, s, eq
+
+
+@pytest.mark.parametrize("name", list(SCEN))
+@pytest.mark.parametrize("bc", [None, ox.DIRICHLET_ROW, ox.DIRICHLET_SYMMETRIC])
+def test_assembly_parity(name, bc):
+    sc = SCEN[name]
+    A, b, Ar, br, ptr, _, _ = _assemble_both(sc, bc, step=min(1, len(sc.steps) - 1))
+    print(f"[assembly] {name} bc={bc} n={sc.n} nnz={ptr[-1]}", flush=True)
+    assert _row_scaled_close(A, Ar, ptr, ENTRY_TOL) < ENTRY_TOL
+    assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
+
+
+@pytest.mark.parametrize("lpr", [4, 8, 16, 32, 64])
+def test_spmv_parity_all_variants(lpr):
+    sc = SCEN["hex"]
+    A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_ROW)
+    s.set_spmv_variant(lpr)
+    x = np.random.default_rng(lpr).standard_normal(sc.n)
+    y = s.spmv(x)
+    eq.a[:] = A  # same matrix on both sides
+    yr = eq.spmv(x)
+    rows = np.repeat(np.arange(sc.n), np.diff(ptr))
+    scale = np.zeros(sc.n)
+    np.add.at(scale, rows, np.abs(A * x[eq.col]))
+    assert np.all(np.abs(y - yr) <= 1e-14 * scale + 1e-300)
+
+
+def _tight_reference(sc):
+    from oracle import ffi
+    return driver.run_oracle(sc, ffi, method=2, symmetric=False, tol=1e-14, keep_steps=True)
+
+
+@pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav"])
+@pytest.mark.parametrize("method,bc,prec", [
+    (ox.CG, ox.DIRICHLET_SYMMETRIC, ox.FP64),
+    (ox.BICGSTAB, ox.DIRICHLET_ROW, ox.FP64),
+    (ox.CG, ox.DIRICHLET_SYMMETRIC, ox.FP32_IR),
+    (ox.BICGSTAB, ox.DIRICHLET_ROW, ox.FP32_IR),
+])
+def test_solution_parity(name, method, bc, prec):
+    sc = SCEN[name]
+    if method == ox.CG and sc.process == ox.HEAT_TRANSPORT:
+        pytest.skip("advection makes A nonsymmetric: CG is not applicable")
+    ref = _tight_reference(sc)
+    res = W.run_gpu(sc, method=method, dirichlet=bc, precision=prec, tol=1e-13, keep_steps=True)
+    print(f"[solve] {name} m={method} bc={bc} prec={prec} "
+          f"iters={[i['iterations'] for i in res['info']]}", flush=True)
+    for k, (x, xr) in enumerate(zip(res["steps"], ref["steps"])):
+        rel = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+        assert rel < SOL_TOL, (k, rel)
+    assert all(i["converged"] in (1, 2) for i in res["info"])
+
+
+def test_same_algorithm_iteration_counts():
+    """GPU CG and BiCGStab follow the reference recurrences: iteration counts
+    match the oracle's on the same system up to rounding-order effects."""
+    from oracle import ffi
+    sc = SCEN["quad"]
+    for method, sym in [(ox.CG, True), (ox.BICGSTAB, False)]:
+        ref = driver.run_oracle(sc, ffi, method=method, symmetric=sym)
+        res = W.run_gpu(sc, method=method,
+                        dirichlet=ox.DIRICHLET_SYMMETRIC if sym else ox.DIRICHLET_ROW)
+        for a, b in zip(res["info"], ref["info"]):
+            assert abs(a["iterations"] - b["iterations"]) <= max(2, 0.03 * b["iterations"])
+            assert a["ret"] == b["ret"] or method == ox.CG
+
+
+def test_bmskel_golden_on_gpu():
+    """The reference's bmskel deck: T(POINT0) = 274 after one step."""
+    sc = SCEN["bmskel"]
+    res = W.run_gpu(sc, method=ox.BICGSTAB, dirichlet=ox.DIRICHLET_ROW)
+    assert res["x"][0] == 274.0
+
+
+def test_write_bin_matches_oracle(tmp_path):
+    from oracle import ffi
+    sc = SCEN["quad"]
+    A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_ROW)
+    s.write_bin(tmp_path / "gpu.bin")
+    eq.write_bin(tmp_path / "ref.bin")
+    n1, p1, c1, a1, b1 = ffi.read_bin(tmp_path / "gpu.bin")
+    n2, p2, c2, a2, b2 = ffi.read_bin(tmp_path / "ref.bin")
+    assert n1 == n2 and np.array_equal(p1, p2) and np.array_equal(c1, c2)
+    assert _row_scaled_close(a1, a2, p2, ENTRY_TOL) < ENTRY_TOL
+    assert np.abs(b1 - b2).max() <= ENTRY_TOL * np.abs(b2).max()
+
+
+def test_deterministic_and_check_norm_rhs():
+    sc = SCEN["hex"]
+    r1 = W.run_gpu(sc, method=ox.CG)
+    r2 = W.run_gpu(sc, method=ox.CG)
+    assert np.array_equal(r1["x"], r2["x"])
+    pat, s = r1["system"]
+    # a right-hand side 1e-12 of the last one is skipped (CheckNormRHS)
+    b = s.get_vector(ox.VEC_B)
+    x0 = s.get_vector(ox.VEC_X)
+    s.set_vector(ox.VEC_B, b * 1e-12)
+    info = s.solve(method=ox.CG, tol=1e-10)
+    assert info["converged"] == 2 and np.array_equal(s.get_vector(ox.VEC_X), x0)
+
+
+def test_max_iter_return_values():
+    sc = SCEN["quad"]
+    for method, ret in [(ox.CG, 0), (ox.BICGSTAB, 4)]:
+        pat, s = W.make_system(sc)
+        s.set_vector(ox.VEC_UOLD, sc.u0)
+        s.initialize()
+        st = sc.steps[0]
+        s.assemble(st.dt, st.theta)
+        s.add_rhs(sc.st_idx, sc.st_val)
+        s.set_dirichlet(sc.bc_idx, sc.bc_val, ox.DIRICHLET_SYMMETRIC)
+        info = s.solve(method=method, tol=1e-10, max_iter=3)
+        assert info["converged"] == 0 and info["ret"] == ret
+
+
+def test_full_size_quad_config_vs_oracle():
+    """The CPU-runnable configuration at full size (200 x 200 quads, 10
+    steps): every step's solution against the oracle's BiCGStab."""
+    from oracle import ffi
+    sc = W.quad_flow()
+    ref = driver.run_oracle(sc, ffi, method=2, symmetric=False, tol=1e-14, keep_steps=True)
+    res = W.run_gpu(sc, method=ox.CG, tol=1e-13, keep_steps=True)
+    for x, xr in zip(res["steps"], ref["steps"]):
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < SOL_TOL
+
+
+def test_full_size_hex_properties():
+    """128^3 hexes (2.1M nodes, 57M nonzeros): size-independent properties.
+    CRS nnz closed form; symmetric elimination leaves A symmetric; the steady
+    operator annihilates constants on interior rows; the FP64 CG and the FP32
+    iterative-refinement solutions have true FP64 residual <= 1e-10 and agree."""
+    import scipy.sparse as sp
+    n = 128
+    sc = W.hex_flow(n, n_transient=0)
+    pat, s = W.make_system(sc)
+    assert pat.nnz == (3 * (n + 1) - 2) ** 3
+    ptr, col, _ = pat.export()
+    st = sc.steps[0]
+    s.set_vector(ox.VEC_UOLD, sc.u0)
+    s.set_vector(ox.VEC_X, sc.u0)
+    s.initialize()
+    s.assemble(st.dt, st.theta, True)
+    A0 = s.get_matrix()
+    csr = sp.csr_matrix((A0, col, ptr), shape=(sc.n, sc.n))
+    rs = np.abs(csr @ np.ones(sc.n))
+    assert rs.max() <= 1e-12 * np.abs(A0).max() * 27
+    s.set_dirichlet(sc.bc_idx, sc.bc_val, ox.DIRICHLET_SYMMETRIC)
+    A = s.get_matrix()
+    b = s.get_vector(ox.VEC_B)
+    csr = sp.csr_matrix((A, col, ptr), shape=(sc.n, sc.n))
+    assert abs(csr - csr.T).max() == 0.0
+    info = s.solve(method=ox.CG, tol=1e-10, max_iter=100000)
+    x64 = s.get_vector(ox.VEC_X)
+    r = np.linalg.norm(b - csr @ x64) / np.linalg.norm(b)
+    print(f"[full hex] iters={info['iterations']} relres={r:.3e}", flush=True)
+    assert info["converged"] == 1 and r <= 1e-10 * 1.5
+    pat2, s2 = W.make_system(sc, ox.FP32_IR)
+    s2.set_vector(ox.VEC_UOLD, sc.u0)
+    s2.set_vector(ox.VEC_X, sc.u0)
+    s2.initialize()
+    s2.assemble(st.dt, st.theta, True)
+    s2.set_dirichlet(sc.bc_idx, sc.bc_val, ox.DIRICHLET_SYMMETRIC)
+    info2 = s2.solve(method=ox.CG, tol=1e-10, max_iter=100000)
+    x32 = s2.get_vector(ox.VEC_X)
+    r2 = np.linalg.norm(b - csr @ x32) / np.linalg.norm(b)
+    print(f"[full hex IR] iters={info2['iterations']} outer={info2['outer']} relres={r2:.3e}",
+          flush=True)
+    assert info2["converged"] == 1 and r2 <= 1e-10
+    assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-6
