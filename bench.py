@@ -1,0 +1,185 @@
+"""Benchmark of the implicit FEM step on the 3-D liquid-flow workload.
+
+Workload: 128^3 trilinear hexes (2,146,689 DOF, 57,066,625 nonzeros),
+lognormal K (sigma_lnK = 2, 8-cell correlation), head 1 m across the x faces.
+One bench step is the whole job of that configuration: a steady solve plus
+5 implicit-Euler transient steps, each = Initialize + element assembly +
+Dirichlet + Jacobi-PCG to ||r||/||b|| <= 1e-10 (true FP64 residual).  It is
+run twice per step: plain FP64 and FP32 matrix/SpMV/preconditioner inside
+FP64 iterative refinement.  Inputs (mesh, K field, pattern) are uploaded to
+HBM before the timed region.
+
+Prints one JSON line.  value = MDOF-solves/s of the faster valid solver
+(both reach 1e-10 FP64 residual); the plain FP64 figures ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def run_job(sc, system, precision):
+    import ogs5_amd as ox
+    from ogs5_amd import workloads as W
+    res = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=precision,
+                    system=system, inner_tol=1e-4)
+    return res["info"]
+
+
+def gpu_leg(sc, precision, steps, warmup):
+    import ogs5_amd as ox
+    from ogs5_amd import workloads as W
+    system = W.make_system(sc, precision)
+    pat, s = system
+    for _ in range(warmup):
+        info = run_job(sc, system, precision)
+        print(f"[bench] warmup precision={precision} iters={[i['iterations'] for i in info]}",
+              file=sys.stderr, flush=True)
+    s.synchronize()
+    s.profile(True)
+    t0 = time.perf_counter()
+    infos = []
+    for _ in range(steps):
+        infos.append(run_job(sc, system, precision))
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"[bench] timed precision={precision} steps={steps} {dt:.3f} s", file=sys.stderr,
+          flush=True)
+    kind = ox._lib.PROF_SPMV64 if precision == ox.FP64 else ox._lib.PROF_SPMV32
+    ms, launches, bpl = s.profile_read(kind)
+    ms64, l64, b64 = s.profile_read(ox._lib.PROF_SPMV64)
+    s.profile(False)
+    solves = len(sc.steps) * steps
+    last = infos[-1]
+    assert all(i["converged"] == 1 for job in infos for i in job), infos[-1]
+    assert all(i["error"] <= sc.tol for job in infos for i in job)
+    return dict(
+        seconds=dt, ms_per_step=1e3 * dt / steps,
+        mdof_solves_per_s=solves * sc.n / dt / 1e6,
+        iterations=[i["iterations"] for i in last], outer=[i["outer"] for i in last],
+        errors=[i["error"] for i in last],
+        spmv=dict(avg_ms=ms / max(launches, 1), launches=launches, bytes_per_launch=bpl,
+                  gbs=bpl / (ms / max(launches, 1)) / 1e6 if launches else None),
+        spmv64=dict(avg_ms=ms64 / max(l64, 1), launches=l64, bytes_per_launch=b64,
+                    gbs=b64 / (ms64 / max(l64, 1)) / 1e6 if l64 else None),
+        spmv_variant=s.spmv_variant(),
+        spmv_tuning=s.spmv_tuning(),
+    )
+
+
+def cpu_baseline(iters_per_solve, n_target, nnz_target, e_target, n_solves):
+    """Serial oracle (the CPU restatement of the reference path) on a bounded
+    sample of the same workload: the 48^3 instance, full assembly and 150
+    Jacobi-CG iterations, timed; the per-element assembly time and the
+    per-nonzero iteration time are scaled to the 128^3 job with the
+    iteration counts the GPU solve needed (same algorithm, same tolerance)."""
+    from ogs5_amd import workloads as W
+    from oracle import ffi
+    m = 48
+    sc = W.hex_flow(m, n_transient=0)
+    ptr, col, diag = ffi.pattern(sc.n, sc.mesh.conn)
+    eq = ffi.OracleEqs(ptr, col, diag)
+    prob = ffi.OracleProblem(sc.mesh, process=sc.process, mass_coef=sc.mass_coef, cond=sc.cond,
+                             dt=sc.steps[0].dt, theta=1.0, steady=True, u_old=sc.u0)
+    t0 = time.perf_counter()
+    eq.initialize()
+    eq.assemble(prob)
+    t_asm = time.perf_counter() - t0
+    # symmetric elimination (untimed) so that CG runs on an SPD system
+    eq.set_known_x(sc.bc_idx, sc.bc_val, symmetric=True)
+    eq.config(5, 1, 1e-30, 150)
+    t0 = time.perf_counter()
+    out = eq.solve()
+    t_it = (time.perf_counter() - t0) / max(out["iterations"], 1)
+    nnz = int(ptr[-1])
+    per_elem = t_asm / sc.mesh.n_elem
+    per_nnz_iter = t_it / nnz
+    total = n_solves * per_elem * e_target + sum(iters_per_solve) * per_nnz_iter * nnz_target
+    return dict(value=n_solves * n_target / total / 1e6, unit="MDOF-solves/s", cores=1,
+                kind="port",
+                sample=(f"oracle (serial C) on the {m}^3 instance: assembly {t_asm:.2f} s for "
+                        f"{sc.mesh.n_elem} hexes, {out['iterations']} CG iterations at "
+                        f"{1e3 * t_it:.1f} ms; scaled per element and per nonzero to the 128^3 "
+                        f"job with the GPU's iteration counts {iters_per_solve}"),
+                assembly_s_per_elem=per_elem, iter_s_per_nnz=per_nnz_iter)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import ogs5_amd as ox
+    from ogs5_amd import workloads as W
+
+    sc = W.hex_flow(args.size)
+    fp64 = gpu_leg(sc, ox.FP64, args.steps, args.warmup)
+    ir = gpu_leg(sc, ox.FP32_IR, args.steps, args.warmup)
+    best = ir if ir["mdof_solves_per_s"] > fp64["mdof_solves_per_s"] else fp64
+    best_name = "fp32_ir" if best is ir else "fp64"
+    sp = best["spmv"]
+    peak = HBM_PEAK_GBS
+    roof = dict(bound="hbm", achieved=sp["gbs"], peak=peak, unit="GB/s",
+                frac=(sp["gbs"] / peak) if sp["gbs"] else None, traffic=None,
+                kernel=("k_cg_spmv<float>" if best is ir else "k_cg_spmv<double>"),
+                avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"])
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        tr = json.load(open(tf)).get(roof["kernel"])
+        if tr:
+            roof["traffic"] = tr
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(fp64["iterations"], sc.n, _nnz(sc), sc.mesh.n_elem, len(sc.steps))
+    out = {
+        "metric": "MDOF-solves/s to 1e-10 FP64 resid; SpMV HBM GB/s; served p50/p99 latency",
+        "value": best["mdof_solves_per_s"],
+        "unit": "MDOF-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best is fp64 else "f32-in-f64-IR",
+        "data": "synthetic (seeded lognormal K, structured hex mesh)",
+        "config": {"workload": f"3D liquid flow, {args.size}^3 hex8, steady + 5 transient "
+                               "steps, Jacobi-PCG to 1e-10",
+                   "dof": sc.n, "nnz": _nnz(sc), "solver": best_name},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": fp64,
+        "fp32_ir": ir,
+        "ir_speedup_vs_fp64": ir["mdof_solves_per_s"] / fp64["mdof_solves_per_s"],
+        "spmv_fp64_hbm_gbs": fp64["spmv"]["gbs"],
+        "served_p50_ms": None,
+        "served_p99_ms": None,
+    }
+    print(json.dumps(out))
+
+
+def _nnz(sc):
+    n1 = sc.mesh.shape[0] + 1
+    return (3 * n1 - 2) ** 3
+
+
+if __name__ == "__main__":
+    main()

@@ -213,10 +213,17 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
 #define OGSX_PROF_SPMV64 0  /* FP64 SpMV kernel                              */
 #define OGSX_PROF_SPMV32 1  /* FP32 SpMV kernel (IR inner solve)             */
 #define OGSX_PROF_ASSEMBLY 2
-/* Chooses the SpMV lanes-per-row variant: 0 = autotune on first use,
- * otherwise 4, 8, 16, 32 or 64. */
-int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row);
-int ogsx_get_spmv_variant(ogsx_system* s, int* lanes_per_row);
+/* SpMV storage/launch variant: 0 = autotune (time every variant on this
+ * matrix at first use and keep the fastest); 1 = tiled CRS (blocks stream
+ * contiguous nonzero tiles through LDS); 2 = tiled CRS with 16-bit column
+ * offsets per tile (when every tile's columns span < 65536); 4, 8, 16, 32,
+ * 64 = vector CRS with that many lanes per row. */
+int ogsx_set_spmv_variant(ogsx_system* s, int variant);
+int ogsx_get_spmv_variant(ogsx_system* s, int* variant);
+/* The autotuner's table: variant and average ms per SpMV launch (FP64 then,
+ * for IR systems, FP32 entries); *count = entries available. */
+int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* variants, double* ms,
+                     int* count);
 /* Synchronises the system's stream. */
 int ogsx_synchronize(ogsx_system* s);
 

@@ -81,6 +81,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_double)]),
     "ogsx_set_spmv_variant": (C.c_int, [_P, C.c_int]),
     "ogsx_get_spmv_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "ogsx_spmv_tuning": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(C.c_int)]),
     "ogsx_synchronize": (C.c_int, [_P]),
 }
 
@@ -259,6 +260,14 @@ class System:
         v = C.c_int()
         check(lib().ogsx_get_spmv_variant(self.h, C.byref(v)), "ogsx_get_spmv_variant")
         return v.value
+
+    def spmv_tuning(self):
+        f = np.zeros(32, np.int32)
+        ms = np.zeros(32, np.float64)
+        cnt = C.c_int()
+        check(lib().ogsx_spmv_tuning(self.h, 32, _ptr(f), _ptr(ms), C.byref(cnt)),
+              "ogsx_spmv_tuning")
+        return [(int(a), float(b)) for a, b in zip(f[:cnt.value], ms[:cnt.value])]
 
     def synchronize(self):
         check(lib().ogsx_synchronize(self.h), "ogsx_synchronize")

@@ -68,7 +68,14 @@ struct ogsx_system {
     int n = 0;
     int64_t nnz = 0;
     int G = 8;
-    int lpr = 0;  // SpMV lanes per row (0 = choose on first use)
+    int fmt64 = 0;  // SpMV format of the FP64 matrix (0 = autotune on first use)
+    int fmt32 = 0;  // SpMV format of the FP32 matrix
+    // row tiles of the tiled SpMV
+    int n_tiles = 0;
+    bool tile16 = false;  // every tile's columns fit 16-bit offsets
+    int *tile_row = nullptr, *tile_cb = nullptr;
+    uint16_t* col16 = nullptr;
+    std::vector<std::pair<int, double>> tune_log;  // (format, ms) of the last tuning
     hipStream_t stream = nullptr;
     // pattern on device
     int *ptr = nullptr, *col = nullptr, *diagpos = nullptr;
@@ -134,12 +141,55 @@ int dalloc(ogsx_system* s, T** p, size_t count)
         if (rc_ != OGSX_OK) return rc_; \
     } while (0)
 
-int choose_lpr(const ogsx_system* s)
+bool valid_fmt(const ogsx_system* s, int f)
 {
-    const double avg = s->n ? (double)s->nnz / s->n : 1.0;
-    int l = 4;
-    while (l < 64 && l < avg) l *= 2;
-    return l;
+    if (f == kFmtTile32) return true;
+    if (f == kFmtTile16) return s->tile16;
+    return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
+}
+
+template <typename VT>
+MatView<VT> mat_view(const ogsx_system* s, const VT* val)
+{
+    return MatView<VT>{s->n, s->n_tiles, s->ptr, s->col, s->col16, val, s->tile_row, s->tile_cb};
+}
+
+// Row tiles: consecutive rows with at most kTileNnz nonzeros and kBlock rows
+// (a row longer than kTileNnz gets a tile of its own); per tile the smallest
+// column, and whether all column offsets fit 16 bits.
+void build_tiles(const PatternHost& h, std::vector<int>& trow, std::vector<int>& tcb,
+                 std::vector<uint16_t>& c16, bool& fits16)
+{
+    trow.clear();
+    tcb.clear();
+    int64_t r = 0;
+    fits16 = true;
+    while (r < h.n) {
+        int64_t r1 = r, nz = 0;
+        while (r1 < h.n && r1 - r < kBlock) {
+            const int64_t len = h.ptr[r1 + 1] - h.ptr[r1];
+            if (nz + len > kTileNnz && r1 > r) break;
+            nz += len;
+            r1++;
+            if (nz > kTileNnz) break;
+        }
+        int64_t lo = INT64_MAX, hi = -1;
+        for (int64_t k = h.ptr[r]; k < h.ptr[r1]; k++) {
+            lo = std::min(lo, h.col[k]);
+            hi = std::max(hi, h.col[k]);
+        }
+        if (hi < 0) lo = hi = 0;
+        if (hi - lo > 65535) fits16 = false;
+        trow.push_back((int)r);
+        tcb.push_back((int)lo);
+        r = r1;
+    }
+    trow.push_back((int)h.n);
+    c16.assign(h.nnz, 0);
+    if (fits16)
+        for (size_t t = 0; t + 1 < trow.size(); t++)
+            for (int64_t k = h.ptr[trow[t]]; k < h.ptr[trow[t + 1]]; k++)
+                c16[k] = (uint16_t)(h.col[k] - tcb[t]);
 }
 
 // ---- profiling helpers ----------------------------------------------------
@@ -184,26 +234,90 @@ int prof_flush(ogsx_system* s)
 }
 
 // ---- SpMV-family launch dispatch on lanes-per-row -------------------------
-#define LPR_DISPATCH(lpr, CALL)          \
-    switch (lpr) {                       \
-        case 4: { constexpr int L = 4; CALL; } break;   \
-        case 8: { constexpr int L = 8; CALL; } break;   \
-        case 16: { constexpr int L = 16; CALL; } break; \
-        case 32: { constexpr int L = 32; CALL; } break; \
-        default: { constexpr int L = 64; CALL; } break; \
+#define FMT_DISPATCH(fmt, CALL)                                 \
+    switch (fmt) {                                              \
+        case kFmtTile32: { constexpr int L = kFmtTile32; CALL; } break; \
+        case kFmtTile16: { constexpr int L = kFmtTile16; CALL; } break; \
+        case 4: { constexpr int L = 4; CALL; } break;           \
+        case 8: { constexpr int L = 8; CALL; } break;           \
+        case 16: { constexpr int L = 16; CALL; } break;         \
+        case 32: { constexpr int L = 32; CALL; } break;         \
+        default: { constexpr int L = 64; CALL; } break;         \
     }
+
+template <typename VT>
+int fmt_of(const ogsx_system* s)
+{
+    return sizeof(VT) == 8 ? s->fmt64 : s->fmt32;
+}
+
+// Autotuner: times every SpMV format (tiled with 16/32-bit columns, vector
+// CRS with 8..64 lanes per row) on this matrix and keeps the fastest.
+template <typename VT, typename T>
+int tune_one(ogsx_system* s, const VT* val, const T* x, T* y, int* fmt_out)
+{
+    const int cands[] = {kFmtTile16, kFmtTile32, 64, 32, 16, 8};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const MatView<VT> mv = mat_view(s, val);
+    int best = kFmtTile32;
+    float best_ms = 1e30f;
+    const int reps = 4;
+    for (int f : cands) {
+        if (!valid_fmt(s, f)) continue;
+        FMT_DISPATCH(f, (k_spmv<L, VT, T><<<s->G, kBlock, 0, s->stream>>>(mv, x, y)));
+        (void)hipEventRecord(e0, s->stream);
+        for (int r = 0; r < reps; r++)
+            FMT_DISPATCH(f, (k_spmv<L, VT, T><<<s->G, kBlock, 0, s->stream>>>(mv, x, y)));
+        (void)hipEventRecord(e1, s->stream);
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        s->tune_log.push_back({f, ms / reps});
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = f;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(hipGetLastError());
+    *fmt_out = best;
+    return OGSX_OK;
+}
+
+int autotune(ogsx_system* s)
+{
+    if (s->fmt64 == 0) {
+        s->tune_log.clear();
+        OK_TRY(tune_one<double, double>(s, s->A, s->v64.x, s->v64.s, &s->fmt64));
+    }
+    if (s->precision == OGSX_FP32_IR && s->fmt32 == 0)
+        OK_TRY(tune_one<float, float>(s, s->A32, s->v32.x, s->v32.s, &s->fmt32));
+    return OGSX_OK;
+}
+
+double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
+{
+    const double n = s->n, nnz = (double)s->nnz;
+    const int sidx = fmt == kFmtTile16 ? 2 : 4;
+    const double tiles = (fmt == kFmtTile16 || fmt == kFmtTile32) ? 8.0 * s->n_tiles : 0.0;
+    return nnz * (sval + sidx) + (n + 1) * 4 + 2 * n * sx + tiles;
+}
 
 template <typename VT, typename T>
 void launch_cg_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, double tol,
                      int prof_kind)
 {
     const int G = s->G;
+    const MatView<VT> mv = mat_view(s, val);
     double* pA = s->part;
     double* pB = s->part + 4 * G;
     for (int it = 0; it < iters; it++) {
         int pk = prof_begin(s, prof_kind);
-        LPR_DISPATCH(s->lpr, (k_cg_spmv<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                 s->n, s->ptr, s->col, val, dv.kv(), pA, s->st)));
+        FMT_DISPATCH(fmt_of<VT>(s), (k_cg_spmv<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                        mv, dv.kv(), pA, s->st)));
         prof_end(s, pk);
         k_cg_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pA, pB, s->st);
         k_cg_direction<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pB, s->st, tol);
@@ -215,6 +329,7 @@ void launch_bi_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, d
                      int prof_kind)
 {
     const int G = s->G;
+    const MatView<VT> mv = mat_view(s, val);
     double* pR = s->part;          // (r,r), (r0,r)
     double* pV = s->part + 4 * G;  // (r0,v)
     double* pS = s->part + 8 * G;  // (s,s)
@@ -222,13 +337,13 @@ void launch_bi_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, d
     for (int it = 0; it < iters; it++) {
         k_bi_dir<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pR, s->st, tol);
         int pk = prof_begin(s, prof_kind);
-        LPR_DISPATCH(s->lpr, (k_bi_spmv_v<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                 s->n, s->ptr, s->col, val, dv.kv(), pV, s->st)));
+        FMT_DISPATCH(fmt_of<VT>(s), (k_bi_spmv_v<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                        mv, dv.kv(), pV, s->st)));
         prof_end(s, pk);
         k_bi_s<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pV, pS, s->st);
         pk = prof_begin(s, prof_kind);
-        LPR_DISPATCH(s->lpr, (k_bi_spmv_t<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                 s->n, s->ptr, s->col, val, dv.kv(), pS, pT, s->st, tol)));
+        FMT_DISPATCH(fmt_of<VT>(s), (k_bi_spmv_t<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                        mv, dv.kv(), pS, pT, s->st, tol)));
         prof_end(s, pk);
         k_bi_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pT, pR, s->st);
     }
@@ -240,13 +355,14 @@ int krylov(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol
            double bnorm_prev, int prof_kind)
 {
     const int G = s->G;
+    const MatView<VT> mv = mat_view(s, val);
     if (method == OGSX_CG) {
-        LPR_DISPATCH(s->lpr, (k_cg_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                 s->n, s->ptr, s->col, val, dv.kv(), s->part)));
+        FMT_DISPATCH(fmt_of<VT>(s), (k_cg_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                        mv, dv.kv(), s->part)));
         k_cg_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, bnorm_prev, tol, max_iter);
     } else {
-        LPR_DISPATCH(s->lpr, (k_bi_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                 s->n, s->ptr, s->col, val, dv.kv(), s->part)));
+        FMT_DISPATCH(fmt_of<VT>(s), (k_bi_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
+                                        mv, dv.kv(), s->part)));
         k_bi_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, bnorm_prev, tol, max_iter);
     }
     HIP_TRY(hipGetLastError());
@@ -423,7 +539,25 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     k_fill<int8_t><<<blocks_for(h.n, kBlock * 4), kBlock, 0, s->stream>>>(h.n, s->bcflag, (int8_t)0);
     if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess)
         return cleanup(fail(OGSX_EHIP, "initialisation kernels failed"));
-    s->lpr = choose_lpr(s);
+    {
+        std::vector<int> trow, tcb;
+        std::vector<uint16_t> c16;
+        build_tiles(h, trow, tcb, c16, s->tile16);
+        s->n_tiles = (int)trow.size() - 1;
+        if ((rc = dalloc(s, &s->tile_row, trow.size())) || (rc = dalloc(s, &s->tile_cb, tcb.size())))
+            return cleanup(rc);
+        if (hipMemcpyAsync(s->tile_row, trow.data(), 4 * trow.size(), hipMemcpyHostToDevice, s->stream) ||
+            hipMemcpyAsync(s->tile_cb, tcb.data(), 4 * tcb.size(), hipMemcpyHostToDevice, s->stream))
+            return cleanup(fail(OGSX_EHIP, "upload of the tiles failed"));
+        if (s->tile16) {
+            if ((rc = dalloc(s, &s->col16, c16.size()))) return cleanup(rc);
+            if (hipMemcpyAsync(s->col16, c16.data(), 2 * c16.size(), hipMemcpyHostToDevice, s->stream) ||
+                hipStreamSynchronize(s->stream))
+                return cleanup(fail(OGSX_EHIP, "upload of the 16-bit columns failed"));
+        }
+        if (hipStreamSynchronize(s->stream) != hipSuccess)
+            return cleanup(fail(OGSX_EHIP, "tile upload failed"));
+    }
     *out = s;
     return OGSX_OK;
 }
@@ -709,6 +843,7 @@ int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* i
     std::memset(&out, 0, sizeof(out));
     const int jac = precond == OGSX_PRECOND_JACOBI;
     const int nb = blocks_for(s->n, kBlock);
+    OK_TRY(autotune(s));
     if (s->precision == OGSX_FP64) {
         k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
         OK_TRY(krylov<double, double>(s, s->A, s->v64, cfg->method, cfg->tol, cfg->max_iter,
@@ -737,8 +872,9 @@ int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* i
         int converged = 0;
         double error = 0.0;
         for (;;) {
-            LPR_DISPATCH(s->lpr, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
-                                     s->n, s->ptr, s->col, s->A, s->v64.x, s->v64.b, s->r64, s->part)));
+            FMT_DISPATCH(s->fmt64, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
+                                       mat_view(s, (const double*)s->A), s->v64.x, s->v64.b,
+                                       s->r64, s->part)));
             k_reduce_to<<<1, kBlock, 0, s->stream>>>(G, s->part, 2, s->red);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(s->red_host, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -839,8 +975,9 @@ int ogsx_spmv(ogsx_system* s, const double* x, double* y)
     // the Krylov work vectors r and s serve as scratch
     HIP_TRY(hipMemcpyAsync(s->v64.r, x, 8 * (size_t)s->n, hipMemcpyHostToDevice, s->stream));
     int pk = prof_begin(s, OGSX_PROF_SPMV64);
-    LPR_DISPATCH(s->lpr, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
-                             s->n, s->ptr, s->col, s->A, s->v64.r, s->v64.s)));
+    OK_TRY(autotune(s));
+    FMT_DISPATCH(s->fmt64, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
+                               mat_view(s, (const double*)s->A), s->v64.r, s->v64.s)));
     prof_end(s, pk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(y, s->v64.s, 8 * (size_t)s->n, hipMemcpyDeviceToHost, s->stream));
@@ -888,11 +1025,10 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms, int64_t* launc
     if (total_ms) *total_ms = s->prof_acc[kind].ms;
     if (launches) *launches = s->prof_acc[kind].launches;
     if (bytes_per_launch) {
-        const double n = s->n, nnz = (double)s->nnz;
         if (kind == OGSX_PROF_SPMV64)
-            *bytes_per_launch = nnz * (8 + 4) + (n + 1) * 4 + n * 8 + n * 8;
+            *bytes_per_launch = spmv_bytes(s, s->fmt64, 8, 8);
         else if (kind == OGSX_PROF_SPMV32)
-            *bytes_per_launch = nnz * (4 + 4) + (n + 1) * 4 + n * 4 + n * 4;
+            *bytes_per_launch = spmv_bytes(s, s->fmt32, 4, 4);
         else
             *bytes_per_launch = 0.0;
     }
@@ -903,20 +1039,34 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
     if (lanes_per_row == 0) {
-        s->lpr = choose_lpr(s);
+        s->fmt64 = s->fmt32 = 0;  // autotune on next use
         return OGSX_OK;
     }
-    if (lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32 &&
-        lanes_per_row != 64)
-        return fail(OGSX_EINVAL, "lanes_per_row must be 4, 8, 16, 32 or 64");
-    s->lpr = lanes_per_row;
+    if (!valid_fmt(s, lanes_per_row))
+        return fail(OGSX_EINVAL, "SpMV variant must be 1 (tiled), 2 (tiled, 16-bit columns; "
+                                 "when the pattern allows), 4, 8, 16, 32 or 64 lanes per row");
+    s->fmt64 = s->fmt32 = lanes_per_row;
     return OGSX_OK;
 }
 
 int ogsx_get_spmv_variant(ogsx_system* s, int* lanes_per_row)
 {
     if (!s || !lanes_per_row) return fail(OGSX_EINVAL, "null argument");
-    *lanes_per_row = s->lpr;
+    OK_TRY(autotune(s));
+    *lanes_per_row = s->fmt64;
+    return OGSX_OK;
+}
+
+int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* formats, double* ms, int* count)
+{
+    if (!s || !count) return fail(OGSX_EINVAL, "null argument");
+    OK_TRY(autotune(s));
+    const int m = (int)s->tune_log.size();
+    for (int i = 0; i < m && i < max_entries; i++) {
+        if (formats) formats[i] = s->tune_log[i].first;
+        if (ms) ms[i] = s->tune_log[i].second;
+    }
+    *count = m;
     return OGSX_OK;
 }
 

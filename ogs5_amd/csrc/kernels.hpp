@@ -134,6 +134,111 @@ __device__ __forceinline__ void vec_chunk(int n, Body&& body)
     for (int i = beg + threadIdx.x; i < end; i += kBlock) body(i);
 }
 
+// Matrix as the SpMV kernels see it: CRS (ptr, col, val) plus the row tiles
+// of the tiled format (tile_row: first row of every tile; tile_cb: the
+// smallest column of the tile, col16 = col - tile_cb when it fits 16 bits).
+template <typename VT>
+struct MatView {
+    int n;
+    int n_tiles;
+    const int* ptr;
+    const int* col;
+    const uint16_t* col16;
+    const VT* val;
+    const int* tile_row;
+    const int* tile_cb;
+};
+
+constexpr int kTileU = 8;                 // nonzeros per thread per tile
+constexpr int kTileNnz = kTileU * kBlock; // nonzeros per tile (2048)
+constexpr int kFmtTile32 = 1;             // tiled, 32-bit columns
+constexpr int kFmtTile16 = 2;             // tiled, 16-bit column offsets
+
+// Tiled CRS SpMV: a block streams the contiguous nonzeros of one tile of
+// rows (<= kTileNnz nonzeros, <= kBlock rows) with kTileU independent
+// coalesced loads per thread, gathers x, parks the products in LDS and then
+// sums each row from LDS with one thread per row.  Many independent loads
+// per thread keep enough bytes in flight to stream at HBM rate, which a
+// lane-group-per-row loop (one dependent load chain per row) does not.
+// Tiles are split into G contiguous ranges dealt XCD-aware like rows.
+// A tile whose single row exceeds kTileNnz is summed by the whole block.
+template <int C16, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __restrict__ x,
+                                           Epi&& epi)
+{
+    __shared__ ACC prod[kTileNnz];
+    int tb, te;
+    chunk_of(m.n_tiles, tb, te);
+    const int tid = threadIdx.x;
+    for (int t = tb; t < te; t++) {
+        const int r0 = m.tile_row[t], r1 = m.tile_row[t + 1];
+        const int k0 = m.ptr[r0], k1 = m.ptr[r1];
+        const int cb = C16 ? m.tile_cb[t] : 0;
+        if (k1 - k0 > kTileNnz) {
+            // one long row (r1 == r0 + 1): block-wide reduction
+            ACC s = 0;
+            for (int k = k0 + tid; k < k1; k += kBlock) {
+                const int c = C16 ? cb + (int)m.col16[k] : m.col[k];
+                s += (ACC)m.val[k] * (ACC)x[c];
+            }
+            double w = wave_sum((double)s);
+            __shared__ double red[kBlock / 64];
+            if ((tid & 63) == 0) red[tid >> 6] = w;
+            __syncthreads();
+            if (tid == 0) {
+                double tot = red[0];
+                for (int q = 1; q < kBlock / 64; q++) tot += red[q];
+                epi(r0, (ACC)tot);
+            }
+            __syncthreads();
+            continue;
+        }
+        VT v[kTileU];
+        int c[kTileU];
+#pragma unroll
+        for (int j = 0; j < kTileU; j++) {
+            const int k = k0 + tid + j * kBlock;
+            if (k < k1) {
+                v[j] = m.val[k];
+                c[j] = C16 ? cb + (int)m.col16[k] : m.col[k];
+            }
+        }
+        XT xv[kTileU];
+#pragma unroll
+        for (int j = 0; j < kTileU; j++) {
+            const int k = k0 + tid + j * kBlock;
+            if (k < k1) xv[j] = x[c[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < kTileU; j++) {
+            const int k = k0 + tid + j * kBlock;
+            if (k < k1) prod[tid + j * kBlock] = (ACC)v[j] * (ACC)xv[j];
+        }
+        __syncthreads();
+        const int r = r0 + tid;
+        if (r < r1) {
+            const int q0 = m.ptr[r] - k0, q1 = m.ptr[r + 1] - k0;
+            ACC s = 0;
+            for (int q = q0; q < q1; q++) s += prod[q];
+            epi(r, s);
+        }
+        __syncthreads();
+    }
+}
+
+// Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled format.
+template <int FMT, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_rows(const MatView<VT>& m, const XT* __restrict__ x,
+                                          Epi&& epi)
+{
+    if constexpr (FMT == kFmtTile32)
+        spmv_tiles<0, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtTile16)
+        spmv_tiles<1, VT, XT, ACC>(m, x, epi);
+    else
+        spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, epi);
+}
+
 // ------------------------------------------------------------------------
 // element assembly (one thread per element of one colour)
 // ------------------------------------------------------------------------
@@ -398,14 +503,12 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
 // ------------------------------------------------------------------------
 // sparse matrix-vector product
 // ------------------------------------------------------------------------
-template <int LPR, typename VT, typename XT>
-__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ ptr,
-                                                 const int* __restrict__ col,
-                                                 const VT* __restrict__ val,
-                                                 const XT* __restrict__ x, XT* __restrict__ y)
+template <int FMT, typename VT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_spmv(MatView<VT> m, const XT* __restrict__ x,
+                                                 XT* __restrict__ y)
 {
     using ACC = XT;
-    spmv_chunk<LPR, VT, XT, ACC>(n, ptr, col, val, x, [&](int r, ACC s) { y[r] = (XT)s; });
+    spmv_rows<FMT, VT, XT, ACC>(m, x, [&](int r, ACC s) { y[r] = (XT)s; });
 }
 
 // ------------------------------------------------------------------------
@@ -429,12 +532,11 @@ struct KVec {
 };
 
 // s = A x; r = b - s; z = r/d; p = z; partials (b,b), (r,r), (r,z)
-template <int LPR, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_init(int n, const int* ptr, const int* col,
-                                                    const VT* val, KVec<T> v, double* part)
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(MatView<VT> m, KVec<T> v, double* part)
 {
     double acc[3] = {0.0, 0.0, 0.0};
-    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.x, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.x, [&](int i, T ax) {
         const T ri = v.b[i] - ax;
         const T zi = ri / v.d[i];
         v.r[i] = ri;
@@ -479,14 +581,13 @@ __global__ void k_cg_init_final(int G, const double* part, KrylovState* st, doub
 }
 
 // K1: s = A p; partial (p,s)
-template <int LPR, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(int n, const int* ptr, const int* col,
-                                                    const VT* val, KVec<T> v, double* part,
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(MatView<VT> m, KVec<T> v, double* part,
                                                     const KrylovState* st)
 {
     if (st->done) return;
     double acc[1] = {0.0};
-    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.p, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.p, [&](int i, T ax) {
         v.s[i] = ax;
         acc[0] += (double)v.p[i] * (double)ax;
     });
@@ -552,12 +653,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, KVec<T> v, const
 // ------------------------------------------------------------------------
 // s = A x; r0 = b - s; r = r0; v = p = 0; partials (r,r), (r0,r), (b,b)
 // (the first two in the layout k_bi_dir reads)
-template <int LPR, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_init(int n, const int* ptr, const int* col,
-                                                    const VT* val, KVec<T> v, double* part)
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_init(MatView<VT> m, KVec<T> v, double* part)
 {
     double acc[3] = {0.0, 0.0, 0.0};
-    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.x, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.x, [&](int i, T ax) {
         const T ri = v.b[i] - ax;
         v.r0[i] = ri;
         v.r[i] = ri;
@@ -650,14 +750,13 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(int n, KVec<T> v, const doubl
 }
 
 // B2: v = A p_h; partial (r0,v); counts the iteration
-template <int LPR, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(int n, const int* ptr, const int* col,
-                                                      const VT* val, KVec<T> v, double* part,
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(MatView<VT> m, KVec<T> v, double* part,
                                                       KrylovState* st)
 {
     if (st->done) return;
     double acc[1] = {0.0};
-    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.ph, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.ph, [&](int i, T ax) {
         v.v[i] = ax;
         acc[0] += (double)v.r0[i] * (double)ax;
     });
@@ -692,12 +791,12 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, KVec<T> v, const double*
 
 // B4: test on s (then x += alpha p_h and stop); else t = A s_h;
 // partials (t,t), (t,s)
-template <int LPR, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(int n, const int* ptr, const int* col,
-                                                      const VT* val, KVec<T> v,
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(MatView<VT> m, KVec<T> v,
                                                       const double* part_ss, double* part,
                                                       KrylovState* st, double tol)
 {
+    const int n = m.n;
     if (st->done) return;
     double ss[1];
     reduce_partials<1>(part_ss, gridDim.x, ss);
@@ -712,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv_t(int n, const int* ptr, con
         return;
     }
     double acc[2] = {0.0, 0.0};
-    spmv_chunk<LPR, VT, T, T>(n, ptr, col, val, v.sh, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.sh, [&](int i, T ax) {
         v.t[i] = ax;
         acc[0] += (double)ax * (double)ax;
         acc[1] += (double)ax * (double)v.s[i];
@@ -748,13 +847,12 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, KVec<T> v, const do
 // FP32-inside-FP64 iterative refinement helpers
 // ------------------------------------------------------------------------
 // r = b - A x in FP64; partials (b,b), (r,r)
-template <int LPR>
-__global__ __launch_bounds__(kBlock) void k_ir_residual(int n, const int* ptr, const int* col,
-                                                        const double* val, const double* x,
+template <int FMT>
+__global__ __launch_bounds__(kBlock) void k_ir_residual(MatView<double> m, const double* x,
                                                         const double* b, double* r, double* part)
 {
     double acc[2] = {0.0, 0.0};
-    spmv_chunk<LPR, double, double, double>(n, ptr, col, val, x, [&](int i, double ax) {
+    spmv_rows<FMT, double, double, double>(m, x, [&](int i, double ax) {
         const double ri = b[i] - ax;
         r[i] = ri;
         acc[0] += b[i] * b[i];

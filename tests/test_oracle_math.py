@@ -303,3 +303,17 @@ def test_write_bin_layout(ffi, tmp_path):
     n, ptr, col, a, bb = ffi.read_bin(p)
     assert n == sc.n and np.array_equal(ptr, r["ptr"]) and np.array_equal(col, r["col"])
     np.testing.assert_array_equal(bb, eq.b)
+
+
+def test_reference_bicgstab_breakdown_on_dirichlet_only_rhs(ffi):
+    """A property of the reference algorithm that the tests rely on: with
+    row-only Dirichlet elimination, x0 = 0 and b nonzero only on Dirichlet
+    rows (a steady solve without sources), one Jacobi-preconditioned step
+    makes s and t vanish on those rows, so (r0, r) = 0 exactly at iteration 2
+    and Linear_EQS::BiCGStab returns 0 with an unconverged x.  CG on the
+    symmetric elimination solves the same system."""
+    sc = W.hex_flow(6, seed=6, n_transient=0)
+    r = driver.run_oracle(sc, ffi, method=2, symmetric=False, tol=1e-13, keep_steps=True)
+    assert r["info"][0]["ret"] == 0 and r["info"][0]["error"] > 0.1
+    c = driver.run_oracle(sc, ffi, method=5, symmetric=True, tol=1e-13)
+    assert c["info"][0]["ret"] == 1 and c["info"][0]["error"] < 1e-13
