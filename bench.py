@@ -32,7 +32,7 @@ def run_job(sc, system, precision):
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
     res = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=precision,
-                    system=system, inner_tol=1e-4)
+                    system=system)
     return res["info"]
 
 

@@ -165,9 +165,18 @@ typedef struct {
     int max_iter;      /* ls_max_iterations                                   */
     double tol;        /* ls_error_tolerance: ||r||/||b|| < tol               */
     /* OGSX_FP32_IR systems only */
-    double inner_tol;  /* relative reduction asked of each FP32 inner solve   */
-    int max_outer;     /* refinement steps                                    */
+    double inner_tol;  /* OGSX_IR_RESTART: relative reduction asked of each
+                          FP32 inner solve; OGSX_IR_RELIABLE: the drop delta
+                          of the FP32 residual that triggers an FP64 update */
+    int max_outer;     /* refinement steps (OGSX_IR_RESTART)                  */
+    int ir_mode;       /* OGSX_IR_RESTART or OGSX_IR_RELIABLE (CG only)       */
 } ogsx_solver_config;
+
+/* FP32_IR refinement schemes */
+#define OGSX_IR_RESTART 0  /* classic: x += solve32(A, b - A x) until done     */
+#define OGSX_IR_RELIABLE 1 /* CG with reliable updates: one FP32 recurrence,
+                              FP64 residual replacement whenever ||r|| drops by
+                              delta; the search direction survives updates    */
 
 typedef struct {
     int iterations;    /* Krylov iterations (inner iterations summed for IR)  */
@@ -213,17 +222,30 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
 #define OGSX_PROF_SPMV64 0  /* FP64 SpMV kernel                              */
 #define OGSX_PROF_SPMV32 1  /* FP32 SpMV kernel (IR inner solve)             */
 #define OGSX_PROF_ASSEMBLY 2
-/* SpMV storage/launch variant: 0 = autotune (time every variant on this
- * matrix at first use and keep the fastest); 1 = tiled CRS (blocks stream
+/* SpMV storage/launch variant: 0 = autotune (time variants 1, 2, 3, 5, 6, 7,
+ * 10, 11 on this matrix at first use and keep the fastest; all of them sum each row in
+ * CRS order over rounded products, bit-identical to CSparseMatrix::multiVec,
+ * so the choice never changes results); 1 = tiled CRS (blocks stream
  * contiguous nonzero tiles through LDS); 2 = tiled CRS with 16-bit column
- * offsets per tile (when every tile's columns span < 65536); 4, 8, 16, 32,
- * 64 = vector CRS with that many lanes per row. */
+ * offsets per tile (when every tile's columns span < 65536); 3 = SELL-64
+ * (a device copy of the CRS values in 64-row slices, column-major, one lane
+ * per row); 5 = SELL-64 with 16-bit column offsets per slice; 6 = as 5 with
+ * 8 entries in flight per lane; 7 = as 6 with non-temporal matrix loads;
+ * 10 = 16 entries in flight, non-temporal; 11 = two slices per wavefront,
+ * non-temporal; 4, 8, 16, 32,
+ * 64 = vector CRS with that many lanes per row (lane-parallel partial sums:
+ * a different rounding order).  The CRS arrays stay the system's matrix;
+ * SELL copies are refreshed after A changes. */
 int ogsx_set_spmv_variant(ogsx_system* s, int variant);
 int ogsx_get_spmv_variant(ogsx_system* s, int* variant);
 /* The autotuner's table: variant and average ms per SpMV launch (FP64 then,
  * for IR systems, FP32 entries); *count = entries available. */
 int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* variants, double* ms,
                      int* count);
+/* Bandwidth calibration: a plain 16-byte-per-lane copy kernel over two
+ * device buffers of `bytes` each, run `reps` times; *gbs = (read + write)
+ * bytes / second / 1e9. */
+int ogsx_stream_copy(int64_t bytes, int reps, double* gbs);
 /* Synchronises the system's stream. */
 int ogsx_synchronize(ogsx_system* s);
 

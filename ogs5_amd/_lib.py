@@ -37,9 +37,13 @@ class Elements(C.Structure):
     ]
 
 
+IR_RESTART, IR_RELIABLE = 0, 1
+
+
 class SolverConfig(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("max_iter", C.c_int),
-                ("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int)]
+                ("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int),
+                ("ir_mode", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -82,6 +86,7 @@ SIGNATURES = {
     "ogsx_set_spmv_variant": (C.c_int, [_P, C.c_int]),
     "ogsx_get_spmv_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "ogsx_spmv_tuning": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(C.c_int)]),
+    "ogsx_stream_copy": (C.c_int, [_I64, C.c_int, C.POINTER(C.c_double)]),
     "ogsx_synchronize": (C.c_int, [_P]),
 }
 
@@ -114,6 +119,13 @@ def device_count() -> int:
     c = C.c_int(0)
     check(lib().ogsx_device_count(C.byref(c)), "ogsx_device_count")
     return c.value
+
+
+def stream_copy_gbs(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Achieved (read + write) GB/s of a plain device copy of nbytes."""
+    g = C.c_double()
+    check(lib().ogsx_stream_copy(int(nbytes), int(reps), C.byref(g)), "ogsx_stream_copy")
+    return g.value
 
 
 def _ptr(a: np.ndarray):
@@ -211,8 +223,14 @@ class System:
               "ogsx_set_dirichlet")
 
     def solve(self, method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
-              inner_tol=1e-4, max_outer=50) -> dict:
-        cfg = SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer)
+              inner_tol=None, max_outer=50, ir_mode=None) -> dict:
+        """IR systems: CG defaults to reliable updates (inner_tol = delta 0.03),
+        BiCGStab to restarted refinement (inner_tol 1e-4)."""
+        if ir_mode is None:
+            ir_mode = IR_RELIABLE if method == CG else IR_RESTART
+        if inner_tol is None:
+            inner_tol = 0.03 if ir_mode == IR_RELIABLE else 1e-4
+        cfg = SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer, ir_mode)
         info = SolveInfo()
         check(lib().ogsx_solve(self.h, C.byref(cfg), C.byref(info)), "ogsx_solve")
         return info.as_dict()

@@ -76,6 +76,16 @@ struct ogsx_system {
     int *tile_row = nullptr, *tile_cb = nullptr;
     uint16_t* col16 = nullptr;
     std::vector<std::pair<int, double>> tune_log;  // (format, ms) of the last tuning
+    // SELL-64 copy of the matrix (values refreshed from A when A changed)
+    int n_slices = 0;
+    int64_t npad = 0;
+    bool sell16 = false;
+    int *sell_ptr = nullptr, *sell_w = nullptr, *sell_cb = nullptr, *sell_c32 = nullptr,
+        *sell_src = nullptr;
+    uint16_t* sell_c16 = nullptr;
+    double* sval64 = nullptr;
+    float* sval32 = nullptr;
+    uint64_t a_version = 1, sval64_version = 0, sval32_version = 0, a32_version = 0;
     hipStream_t stream = nullptr;
     // pattern on device
     int *ptr = nullptr, *col = nullptr, *diagpos = nullptr;
@@ -143,15 +153,97 @@ int dalloc(ogsx_system* s, T** p, size_t count)
 
 bool valid_fmt(const ogsx_system* s, int f)
 {
-    if (f == kFmtTile32) return true;
+    if (f == kFmtTile32 || f == kFmtSell32) return true;
     if (f == kFmtTile16) return s->tile16;
+    if (f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT || f == kFmtSell16U16 ||
+        f == kFmtSell16X2)
+        return s->sell16;
     return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
 }
 
 template <typename VT>
 MatView<VT> mat_view(const ogsx_system* s, const VT* val)
 {
-    return MatView<VT>{s->n, s->n_tiles, s->ptr, s->col, s->col16, val, s->tile_row, s->tile_cb};
+    const VT* sv = nullptr;
+    if constexpr (sizeof(VT) == 8)
+        sv = s->sval64;
+    else
+        sv = s->sval32;
+    return MatView<VT>{s->n,        s->n_tiles,  s->ptr,      s->col,     s->col16,
+                       val,         s->tile_row, s->tile_cb,  s->n_slices, s->sell_ptr,
+                       s->sell_w,   s->sell_cb,  s->sell_c16, s->sell_c32, sv};
+}
+
+inline bool is_sell(int f)
+{
+    return f == kFmtSell32 || f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT ||
+           f == kFmtSell16U16 || f == kFmtSell16X2;
+}
+
+// SELL-64 layout of the pattern: slice widths, offsets, columns (32-bit and,
+// when every slice's columns span < 65536, 16-bit offsets from the slice's
+// smallest column) and the CRS position of every SELL entry (-1 = padding).
+void build_sell(const PatternHost& h, std::vector<int>& sptr, std::vector<int>& sw,
+                std::vector<int>& scb, std::vector<int>& c32, std::vector<uint16_t>& c16,
+                std::vector<int>& src, bool& fits16)
+{
+    const int64_t ns = (h.n + kSellC - 1) / kSellC;
+    sptr.assign(ns + 1, 0);
+    sw.assign(ns, 0);
+    scb.assign(ns, 0);
+    fits16 = true;
+    for (int64_t sl = 0; sl < ns; sl++) {
+        int w = 0;
+        int64_t lo = INT64_MAX, hi = -1;
+        for (int64_t r = sl * kSellC; r < std::min<int64_t>(h.n, (sl + 1) * kSellC); r++) {
+            w = std::max<int>(w, (int)(h.ptr[r + 1] - h.ptr[r]));
+            for (int64_t k = h.ptr[r]; k < h.ptr[r + 1]; k++) {
+                lo = std::min(lo, h.col[k]);
+                hi = std::max(hi, h.col[k]);
+            }
+        }
+        if (hi < 0) lo = hi = 0;
+        if (hi - lo > 65535) fits16 = false;
+        sw[sl] = w;
+        scb[sl] = (int)lo;
+        sptr[sl + 1] = sptr[sl] + w * kSellC;
+    }
+    const int64_t npad = sptr[ns];
+    c32.assign(npad, 0);
+    src.assign(npad, -1);
+    for (int64_t sl = 0; sl < ns; sl++)
+        for (int l = 0; l < kSellC; l++) {
+            const int64_t r = sl * kSellC + l;
+            for (int j = 0; j < sw[sl]; j++) {
+                const int64_t p = sptr[sl] + (int64_t)j * kSellC + l;
+                if (r < h.n && h.ptr[r] + j < h.ptr[r + 1]) {
+                    c32[p] = (int)h.col[h.ptr[r] + j];
+                    src[p] = (int)(h.ptr[r] + j);
+                } else {
+                    c32[p] = scb[sl];  // padding: any valid column, value 0
+                }
+            }
+        }
+    c16.assign(fits16 ? npad : 0, 0);
+    if (fits16)
+        for (int64_t sl = 0; sl < ns; sl++)
+            for (int64_t p = sptr[sl]; p < sptr[sl + 1]; p++) c16[p] = (uint16_t)(c32[p] - scb[sl]);
+}
+
+// Brings the SELL value copies up to date with A (after assembly / BC).
+int sync_sell(ogsx_system* s, bool want64, bool want32)
+{
+    const int nb = blocks_for(s->npad, kBlock * 4);
+    if (want64 && s->sval64_version != s->a_version) {
+        k_to_sell<double><<<nb, kBlock, 0, s->stream>>>(s->npad, s->sell_src, s->A, s->sval64);
+        s->sval64_version = s->a_version;
+    }
+    if (want32 && s->sval32 && s->sval32_version != s->a_version) {
+        k_to_sell<float><<<nb, kBlock, 0, s->stream>>>(s->npad, s->sell_src, s->A, s->sval32);
+        s->sval32_version = s->a_version;
+    }
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
 }
 
 // Row tiles: consecutive rows with at most kTileNnz nonzeros and kBlock rows
@@ -238,6 +330,12 @@ int prof_flush(ogsx_system* s)
     switch (fmt) {                                              \
         case kFmtTile32: { constexpr int L = kFmtTile32; CALL; } break; \
         case kFmtTile16: { constexpr int L = kFmtTile16; CALL; } break; \
+        case kFmtSell32: { constexpr int L = kFmtSell32; CALL; } break; \
+        case kFmtSell16: { constexpr int L = kFmtSell16; CALL; } break; \
+        case kFmtSell16U8: { constexpr int L = kFmtSell16U8; CALL; } break; \
+        case kFmtSell16NT: { constexpr int L = kFmtSell16NT; CALL; } break; \
+        case kFmtSell16U16: { constexpr int L = kFmtSell16U16; CALL; } break; \
+        case kFmtSell16X2: { constexpr int L = kFmtSell16X2; CALL; } break; \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -256,7 +354,10 @@ int fmt_of(const ogsx_system* s)
 template <typename VT, typename T>
 int tune_one(ogsx_system* s, const VT* val, const T* x, T* y, int* fmt_out)
 {
-    const int cands[] = {kFmtTile16, kFmtTile32, 64, 32, 16, 8};
+    // Only formats that sum every row in CRS order are candidates, so the
+    // choice (which depends on timing) never changes the results.
+    const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
+                         kFmtSell16X2,  kFmtSell32,   kFmtTile16,   kFmtTile32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -301,14 +402,19 @@ int autotune(ogsx_system* s)
 double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
 {
     const double n = s->n, nnz = (double)s->nnz;
-    const int sidx = fmt == kFmtTile16 ? 2 : 4;
+    const int sidx = (fmt == kFmtTile16 || fmt == kFmtSell16 || fmt == kFmtSell16U8 ||
+                      fmt == kFmtSell16NT || fmt == kFmtSell16U16 || fmt == kFmtSell16X2)
+                         ? 2
+                         : 4;
+    if (is_sell(fmt))  // stored entries incl. padding, slice table (ptr, width, base)
+        return (double)s->npad * (sval + sidx) + 12.0 * s->n_slices + 2 * n * sx;
     const double tiles = (fmt == kFmtTile16 || fmt == kFmtTile32) ? 8.0 * s->n_tiles : 0.0;
     return nnz * (sval + sidx) + (n + 1) * 4 + 2 * n * sx + tiles;
 }
 
 template <typename VT, typename T>
 void launch_cg_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, double tol,
-                     int prof_kind)
+                     int prof_kind, double upd_thresh)
 {
     const int G = s->G;
     const MatView<VT> mv = mat_view(s, val);
@@ -320,7 +426,7 @@ void launch_cg_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, d
                                         mv, dv.kv(), pA, s->st)));
         prof_end(s, pk);
         k_cg_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pA, pB, s->st);
-        k_cg_direction<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pB, s->st, tol);
+        k_cg_direction<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pB, s->st, tol, upd_thresh);
     }
 }
 
@@ -349,6 +455,31 @@ void launch_bi_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, d
     }
 }
 
+// Launches chunks of iterations until the state leaves "running"; the chunk
+// grows from 4 to 64 iterations between host checks.
+template <typename VT, typename T>
+int krylov_loop(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol,
+                int max_iter, int prof_kind, double upd_thresh, int* launched)
+{
+    int chunk = 4;
+    while (!s->st_host->done) {
+        const int c = std::min(chunk, max_iter + 1 - *launched);
+        if (c <= 0) break;
+        if (method == OGSX_CG)
+            launch_cg_chunk<VT, T>(s, val, dv, c, tol, prof_kind, upd_thresh);
+        else
+            launch_bi_chunk<VT, T>(s, val, dv, c, tol, prof_kind);
+        *launched += c;
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                               s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        OK_TRY(prof_flush(s));
+        chunk = std::min(chunk * 2, 64);
+    }
+    return OGSX_OK;
+}
+
 // Runs one Krylov solve on (val, dv) from dv.x; fills st_host.
 template <typename VT, typename T>
 int krylov(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol, int max_iter,
@@ -369,24 +500,79 @@ int krylov(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol
     HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    int chunk = 4;
     int launched = 0;
-    while (!s->st_host->done) {
-        const int c = std::min(chunk, max_iter + 1 - launched);
-        if (c <= 0) break;
-        if (method == OGSX_CG)
-            launch_cg_chunk<VT, T>(s, val, dv, c, tol, prof_kind);
-        else
-            launch_bi_chunk<VT, T>(s, val, dv, c, tol, prof_kind);
-        launched += c;
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
-                               s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        OK_TRY(prof_flush(s));
-        chunk = std::min(chunk * 2, 64);
-    }
+    return krylov_loop<VT, T>(s, val, dv, method, tol, max_iter, prof_kind, 0.0, &launched);
+}
+
+// FP64 residual r64 = b - A x64 and its norms (host copies in red_host[0..1]).
+int ir_residual(ogsx_system* s)
+{
+    const int G = s->G;
+    FMT_DISPATCH(s->fmt64, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
+                               mat_view(s, (const double*)s->A), s->v64.x, s->v64.b, s->r64,
+                               s->part)));
+    k_reduce_to<<<1, kBlock, 0, s->stream>>>(G, s->part, 2, s->red);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->red_host, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return OGSX_OK;
+}
+
+// Mixed-precision CG with reliable updates: the Jacobi-CG recurrence runs in
+// FP32 (matrix, vectors, SpMV, preconditioner) on r scaled by sigma =
+// 1/||r0||; whenever the FP32 residual falls by delta since the last update
+// (or claims convergence), the FP32 iterate is folded into the FP64 x, the
+// true residual b - A x is recomputed in FP64, and the recurrence resumes
+// from it with the same search direction (Clark et al., Comput. Phys.
+// Commun. 181 (2010) 1517, "reliable updates").  The stopping test is the
+// FP64 residual.
+int reliable_cg(ogsx_system* s, const ogsx_solver_config* cfg, double delta, double bnorm,
+                double rnorm0, int* total, int* updates, double* error, int* converged)
+{
+    const int G = s->G;
+    const double sigma = 1.0 / rnorm0;
+    const double tol_in = cfg->tol * bnorm * sigma;  // FP64 target in scaled units
+    double maxr = 1.0;
+    int launched = 0;
+    k_ir_prepare<<<G, kBlock, 0, s->stream>>>(s->n, s->r64, sigma, s->v32.b, s->v32.x);
+    const MatView<float> mv = mat_view(s, (const float*)s->A32);
+    FMT_DISPATCH(s->fmt32, (k_cg_init<L, float, float><<<G, kBlock, 0, s->stream>>>(
+                               mv, s->v32.kv(), s->part)));
+    k_cg_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, 0.0, tol_in, cfg->max_iter);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (;;) {
+        OK_TRY(krylov_loop<float, float>(s, s->A32, s->v32, OGSX_CG, tol_in, cfg->max_iter,
+                                         OGSX_PROF_SPMV32, delta * maxr, &launched));
+        const KrylovState st = *s->st_host;
+        *total = st.iter;
+        // fold the FP32 iterate into x and recompute the FP64 residual
+        k_ir_correct<<<G, kBlock, 0, s->stream>>>(s->n, s->v64.x, s->v32.x, rnorm0);
+        OK_TRY(ir_residual(s));
+        (*updates)++;
+        const double rn = std::sqrt(s->red_host[1]);
+        *error = rn / bnorm;
+        if (*error < cfg->tol) {
+            *converged = 1;
+            return OGSX_OK;
+        }
+        if (st.done == 2 || st.iter >= cfg->max_iter) {
+            *converged = 0;
+            return OGSX_OK;
+        }
+        if (st.done == 3) {
+            *converged = 3;
+            return OGSX_OK;
+        }
+        // resume: r32 = sigma r64, x32 = 0, then the interrupted direction step
+        k_ir_prepare<<<G, kBlock, 0, s->stream>>>(s->n, s->r64, sigma, s->v32.r, s->v32.x);
+        k_cg_rescan<float><<<G, kBlock, 0, s->stream>>>(s->n, s->v32.kv(), s->part);
+        k_cg_resume<float><<<G, kBlock, 0, s->stream>>>(s->n, s->v32.kv(), s->part, s->st);
+        HIP_TRY(hipGetLastError());
+        maxr = rn * sigma;
+        s->st_host->done = 0;
+    }
 }
 
 template <typename T>
@@ -558,6 +744,25 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
         if (hipStreamSynchronize(s->stream) != hipSuccess)
             return cleanup(fail(OGSX_EHIP, "tile upload failed"));
     }
+    {
+        std::vector<int> sptr, sw, scb, c32, src;
+        std::vector<uint16_t> c16;
+        build_sell(h, sptr, sw, scb, c32, c16, src, s->sell16);
+        s->n_slices = (int)sw.size();
+        s->npad = sptr.back();
+        if ((rc = dalloc(s, &s->sell_ptr, sptr.size())) || (rc = dalloc(s, &s->sell_w, sw.size())) ||
+            (rc = dalloc(s, &s->sell_cb, scb.size())) || (rc = dalloc(s, &s->sell_c32, c32.size())) ||
+            (rc = dalloc(s, &s->sell_src, src.size())) || (rc = dalloc(s, &s->sval64, s->npad)))
+            return cleanup(rc);
+        if (precision == OGSX_FP32_IR && (rc = dalloc(s, &s->sval32, s->npad))) return cleanup(rc);
+        if (s->sell16 && (rc = dalloc(s, &s->sell_c16, c16.size()))) return cleanup(rc);
+        if (up(s->sell_ptr, sptr.data(), 4 * sptr.size()) || up(s->sell_w, sw.data(), 4 * sw.size()) ||
+            up(s->sell_cb, scb.data(), 4 * scb.size()) || up(s->sell_c32, c32.data(), 4 * c32.size()) ||
+            up(s->sell_src, src.data(), 4 * src.size()) ||
+            (s->sell16 && up(s->sell_c16, c16.data(), 2 * c16.size())) ||
+            hipStreamSynchronize(s->stream))
+            return cleanup(fail(OGSX_EHIP, "upload of the SELL layout failed"));
+    }
     *out = s;
     return OGSX_OK;
 }
@@ -687,6 +892,7 @@ int ogsx_system_initialize(ogsx_system* s)
     if (!s) return fail(OGSX_EINVAL, "null system");
     k_fill<double><<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, 0.0);
     k_fill<double><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v64.b, 0.0);
+    s->a_version++;
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
 }
@@ -732,6 +938,7 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
         default: return fail(OGSX_EINVAL, "unsupported element type");
     }
     prof_end(s, pk);
+    s->a_version++;
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
 }
@@ -826,6 +1033,7 @@ int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx, const doub
         k_bc_mark<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d,
                                                                    s->bcflag, s->bcg, (int8_t)0);
     }
+    s->a_version++;
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
 }
@@ -844,6 +1052,7 @@ int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* i
     const int jac = precond == OGSX_PRECOND_JACOBI;
     const int nb = blocks_for(s->n, kBlock);
     OK_TRY(autotune(s));
+    OK_TRY(sync_sell(s, is_sell(s->fmt64), s->precision == OGSX_FP32_IR && is_sell(s->fmt32)));
     if (s->precision == OGSX_FP64) {
         k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
         OK_TRY(krylov<double, double>(s, s->A, s->v64, cfg->method, cfg->tol, cfg->max_iter,
@@ -864,13 +1073,17 @@ int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* i
         const int G = s->G;
         k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
         k_diag<float><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v32.d, jac);
-        k_cast_f32<<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, s->A32);
+        if (!is_sell(s->fmt32) && s->a32_version != s->a_version) {
+            k_cast_f32<<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, s->A32);
+            s->a32_version = s->a_version;
+        }
         const double inner_tol = cfg->inner_tol > 0.0 ? cfg->inner_tol : 1e-4;
         const int max_outer = cfg->max_outer > 0 ? cfg->max_outer : 50;
         double bnorm_new = 0.0;
         int total = 0, outer = 0;
         int converged = 0;
         double error = 0.0;
+        const bool reliable = cfg->method == OGSX_CG && cfg->ir_mode == OGSX_IR_RELIABLE;
         for (;;) {
             FMT_DISPATCH(s->fmt64, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
                                        mat_view(s, (const double*)s->A), s->v64.x, s->v64.b,
@@ -898,6 +1111,12 @@ int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* i
             error = rnorm / bnorm_new;
             if (error < cfg->tol) {
                 converged = 1;
+                break;
+            }
+            if (reliable) {
+                const double delta = cfg->inner_tol > 0.0 ? cfg->inner_tol : 0.03;
+                OK_TRY(reliable_cg(s, cfg, delta, bnorm_new, rnorm, &total, &outer, &error,
+                                   &converged));
                 break;
             }
             if (outer >= max_outer || total >= cfg->max_iter) {
@@ -976,6 +1195,7 @@ int ogsx_spmv(ogsx_system* s, const double* x, double* y)
     HIP_TRY(hipMemcpyAsync(s->v64.r, x, 8 * (size_t)s->n, hipMemcpyHostToDevice, s->stream));
     int pk = prof_begin(s, OGSX_PROF_SPMV64);
     OK_TRY(autotune(s));
+    OK_TRY(sync_sell(s, is_sell(s->fmt64), false));
     FMT_DISPATCH(s->fmt64, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
                                mat_view(s, (const double*)s->A), s->v64.r, s->v64.s)));
     prof_end(s, pk);
@@ -1043,8 +1263,9 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
         return OGSX_OK;
     }
     if (!valid_fmt(s, lanes_per_row))
-        return fail(OGSX_EINVAL, "SpMV variant must be 1 (tiled), 2 (tiled, 16-bit columns; "
-                                 "when the pattern allows), 4, 8, 16, 32 or 64 lanes per row");
+        return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
+                                 "3/5/6/7/10/11 (SELL-64), or 4, 8, 16, 32, 64 lanes per row "
+                                 "(16-bit forms only when the pattern allows)");
     s->fmt64 = s->fmt32 = lanes_per_row;
     return OGSX_OK;
 }
@@ -1067,6 +1288,40 @@ int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* formats, double* ms, 
         if (ms) ms[i] = s->tune_log[i].second;
     }
     *count = m;
+    return OGSX_OK;
+}
+
+int ogsx_stream_copy(int64_t bytes, int reps, double* gbs)
+{
+    if (bytes < 16 || reps < 1 || !gbs) return fail(OGSX_EINVAL, "bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(OGSX_EHIP, "no HIP device available");
+    const int64_t n4 = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, n4 * 16));
+    if (hipMalloc(&b, n4 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(OGSX_ENOMEM, "hipMalloc failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int nb = 2048 * 8;
+    (void)hipMemset(a, 0, n4 * 16);
+    k_stream_copy<<<nb, kBlock>>>(n4, a, b);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; r++) k_stream_copy<<<nb, kBlock>>>(n4, a, b);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *gbs = 2.0 * n4 * 16 * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    HIP_TRY(hipGetLastError());
     return OGSX_OK;
 }
 

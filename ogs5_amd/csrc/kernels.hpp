@@ -147,12 +147,30 @@ struct MatView {
     const VT* val;
     const int* tile_row;
     const int* tile_cb;
+    // SELL-64: slice s holds rows [64 s, 64 s + 64) column-major: entry j of
+    // lane l at sell_ptr[s] + 64 j + l, j < sell_w[s]; padding has value 0
+    // and column offset 0.  Columns are sell_cb[s] + sell_c16 (16-bit) or
+    // sell_c32.
+    int n_slices;
+    const int* sell_ptr;
+    const int* sell_w;
+    const int* sell_cb;
+    const uint16_t* sell_c16;
+    const int* sell_c32;
+    const VT* sval;
 };
 
 constexpr int kTileU = 8;                 // nonzeros per thread per tile
 constexpr int kTileNnz = kTileU * kBlock; // nonzeros per tile (2048)
 constexpr int kFmtTile32 = 1;             // tiled, 32-bit columns
 constexpr int kFmtTile16 = 2;             // tiled, 16-bit column offsets
+constexpr int kFmtSell32 = 3;             // SELL-64, 32-bit columns
+constexpr int kFmtSell16 = 5;             // SELL-64, 16-bit column offsets
+constexpr int kFmtSell16U8 = 6;           // ... 8 entries in flight per lane
+constexpr int kFmtSell16NT = 7;           // ... and non-temporal matrix loads
+constexpr int kFmtSell16U16 = 10;         // SELL-64, 16-bit, 16 in flight, NT
+constexpr int kFmtSell16X2 = 11;          // SELL-64, 16-bit, two slices per wave, NT
+constexpr int kSellC = 64;                // rows per slice = wavefront size
 
 // Tiled CRS SpMV: a block streams the contiguous nonzeros of one tile of
 // rows (<= kTileNnz nonzeros, <= kBlock rows) with kTileU independent
@@ -160,12 +178,14 @@ constexpr int kFmtTile16 = 2;             // tiled, 16-bit column offsets
 // sums each row from LDS with one thread per row.  Many independent loads
 // per thread keep enough bytes in flight to stream at HBM rate, which a
 // lane-group-per-row loop (one dependent load chain per row) does not.
-// Tiles are split into G contiguous ranges dealt XCD-aware like rows.
+// Tiles are split into G contiguous ranges dealt XCD-aware like rows.  Rows
+// are summed left to right over rounded products, as the SELL kernel does.
 // A tile whose single row exceeds kTileNnz is summed by the whole block.
 template <int C16, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __restrict__ x,
                                            Epi&& epi)
 {
+#pragma clang fp contract(off)
     __shared__ ACC prod[kTileNnz];
     int tb, te;
     chunk_of(m.n_tiles, tb, te);
@@ -179,7 +199,8 @@ __device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __res
             ACC s = 0;
             for (int k = k0 + tid; k < k1; k += kBlock) {
                 const int c = C16 ? cb + (int)m.col16[k] : m.col[k];
-                s += (ACC)m.val[k] * (ACC)x[c];
+                const ACC pr = (ACC)m.val[k] * (ACC)x[c];
+                s = s + pr;
             }
             double w = wave_sum((double)s);
             __shared__ double red[kBlock / 64];
@@ -226,7 +247,127 @@ __device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __res
     }
 }
 
-// Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled format.
+// SELL-64 SpMV: one wavefront per slice, one lane per row.  Each row is
+// summed left to right from 0 over rounded products (no FMA contraction):
+// the order and rounding of CSparseMatrix::multiVec (matrix_class.cpp:1527-
+// 1538), so the result is bit-identical to the reference's SpMV and to the
+// tiled kernel, whichever of them the autotuner picks.  Loads of the
+// values and column offsets are fully coalesced (64 consecutive entries per
+// instruction), rows need no cross-lane reduction, and for meshes numbered
+// along a grid line the 64 lanes' x gathers of one entry j touch 64
+// consecutive x values.  Blocks take contiguous slice ranges, XCD-aware.
+template <typename T>
+__device__ __forceinline__ T ld(const T* p, bool nt)
+{
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+template <int C16, int U, bool NT, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_sell(const MatView<VT>& m, const XT* __restrict__ x,
+                                          Epi&& epi)
+{
+#pragma clang fp contract(off)
+    int sb, se;
+    chunk_of(m.n_slices, sb, se);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int sl = sb + wave; sl < se; sl += kBlock / 64) {
+        const int base = m.sell_ptr[sl] + lane;
+        const int w = m.sell_w[sl];
+        const int cb = C16 ? m.sell_cb[sl] : 0;
+        ACC s = 0;
+        int j = 0;
+        for (; j + U <= w; j += U) {
+            int c[U];
+            VT v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = base + (j + u) * kSellC;
+                c[u] = C16 ? cb + (int)ld(m.sell_c16 + k, NT) : ld(m.sell_c32 + k, NT);
+                v[u] = ld(m.sval + k, NT);
+            }
+            ACC xv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) xv[u] = (ACC)x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const ACC prod = (ACC)v[u] * xv[u];
+                s = s + prod;
+            }
+        }
+        for (; j < w; j++) {
+            const int k = base + j * kSellC;
+            const int c0 = C16 ? cb + (int)m.sell_c16[k] : m.sell_c32[k];
+            const ACC prod = (ACC)m.sval[k] * (ACC)x[c0];
+            s = s + prod;
+        }
+        const int row = sl * kSellC + lane;
+        if (row < m.n) epi(row, s);
+    }
+}
+
+// Two slices per wavefront in one loop: twice the independent loads in
+// flight; each row is still summed in CRS order.
+template <int U, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_sell_x2(const MatView<VT>& m, const XT* __restrict__ x,
+                                             Epi&& epi)
+{
+#pragma clang fp contract(off)
+    int sb, se;
+    chunk_of(m.n_slices, sb, se);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int W = kBlock / 64;
+    for (int sl = sb + wave; sl < se; sl += 2 * W) {
+        const int sl2 = sl + W;
+        const bool two = sl2 < se;
+        const int base0 = m.sell_ptr[sl] + lane, w0 = m.sell_w[sl], cb0 = m.sell_cb[sl];
+        const int base1 = two ? m.sell_ptr[sl2] + lane : 0, w1 = two ? m.sell_w[sl2] : 0;
+        const int cb1 = two ? m.sell_cb[sl2] : 0;
+        ACC s0 = 0, s1 = 0;
+        const int wmax = w0 > w1 ? w0 : w1;
+        for (int j = 0; j < wmax; j += U) {
+            int c0[U], c1[U];
+            VT v0[U], v1[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k0 = base0 + (j + u) * kSellC, k1 = base1 + (j + u) * kSellC;
+                if (j + u < w0) {
+                    c0[u] = cb0 + (int)__builtin_nontemporal_load(m.sell_c16 + k0);
+                    v0[u] = __builtin_nontemporal_load(m.sval + k0);
+                }
+                if (j + u < w1) {
+                    c1[u] = cb1 + (int)__builtin_nontemporal_load(m.sell_c16 + k1);
+                    v1[u] = __builtin_nontemporal_load(m.sval + k1);
+                }
+            }
+            ACC x0[U], x1[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (j + u < w0) x0[u] = (ACC)x[c0[u]];
+                if (j + u < w1) x1[u] = (ACC)x[c1[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (j + u < w0) {
+                    const ACC p0 = (ACC)v0[u] * x0[u];
+                    s0 = s0 + p0;
+                }
+                if (j + u < w1) {
+                    const ACC p1 = (ACC)v1[u] * x1[u];
+                    s1 = s1 + p1;
+                }
+            }
+        }
+        const int row0 = sl * kSellC + lane;
+        if (row0 < m.n) epi(row0, s0);
+        if (two) {
+            const int row1 = sl2 * kSellC + lane;
+            if (row1 < m.n) epi(row1, s1);
+        }
+    }
+}
+
+// Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled or
+// SELL-64 format.
 template <int FMT, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_rows(const MatView<VT>& m, const XT* __restrict__ x,
                                           Epi&& epi)
@@ -235,8 +376,33 @@ __device__ __forceinline__ void spmv_rows(const MatView<VT>& m, const XT* __rest
         spmv_tiles<0, VT, XT, ACC>(m, x, epi);
     else if constexpr (FMT == kFmtTile16)
         spmv_tiles<1, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtSell32)
+        spmv_sell<0, 4, false, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtSell16)
+        spmv_sell<1, 4, false, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtSell16U8)
+        spmv_sell<1, 8, false, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtSell16NT)
+        spmv_sell<1, 8, true, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtSell16U16)
+        spmv_sell<1, 16, true, VT, XT, ACC>(m, x, epi);
+    else if constexpr (FMT == kFmtSell16X2)
+        spmv_sell_x2<4, VT, XT, ACC>(m, x, epi);
     else
         spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, epi);
+}
+
+// CRS values -> SELL-64 values (padding 0): sval[p] = A[src[p]].
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_to_sell(int64_t npad, const int* __restrict__ src,
+                                                    const double* __restrict__ A,
+                                                    VT* __restrict__ sval)
+{
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < npad;
+         p += (int64_t)gridDim.x * kBlock) {
+        const int k = src[p];
+        sval[p] = k >= 0 ? (VT)A[k] : (VT)0;
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -605,14 +771,45 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, KVec<T> v, const do
     const double rr = st->rr;
     const T alpha = (T)(rr / ps[0]);
     double acc[2] = {0.0, 0.0};
-    vec_chunk(n, [&](int i) {
-        v.x[i] += alpha * v.p[i];
-        const T ri = v.r[i] - alpha * v.s[i];
-        v.r[i] = ri;
-        const T zi = ri / v.d[i];
-        acc[0] += (double)ri * (double)ri;
-        acc[1] += (double)zi * (double)ri;
-    });
+    T* __restrict__ x = v.x;
+    T* __restrict__ r = v.r;
+    const T* __restrict__ p = v.p;
+    const T* __restrict__ s = v.s;
+    const T* __restrict__ d = v.d;
+    int beg, end;
+    chunk_of(n, beg, end);
+    constexpr int U = 4;
+    int i = beg + threadIdx.x;
+    for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
+        T xi[U], pi[U], ri[U], si[U], di[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            xi[u] = x[k];
+            pi[u] = p[k];
+            ri[u] = r[k];
+            si[u] = s[k];
+            di[u] = d[k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            x[k] = xi[u] + alpha * pi[u];
+            const T rn = ri[u] - alpha * si[u];
+            r[k] = rn;
+            const T zi = rn / di[u];
+            acc[0] += (double)rn * (double)rn;
+            acc[1] += (double)zi * (double)rn;
+        }
+    }
+    for (; i < end; i += kBlock) {
+        x[i] += alpha * p[i];
+        const T rn = r[i] - alpha * s[i];
+        r[i] = rn;
+        const T zi = rn / d[i];
+        acc[0] += (double)rn * (double)rn;
+        acc[1] += (double)zi * (double)rn;
+    }
     store_partials<2>(acc, part);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->rr_old = rr;
@@ -620,31 +817,87 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, KVec<T> v, const do
     }
 }
 
-// K3: error test; beta = (z,r)/rr_old; p = r/d + beta p
+// K3: error test; beta = (z,r)/rr_old; p = r/d + beta p.  upd_thresh > 0
+// (reliable-update mixed precision) stops the recurrence with done = 5 when
+// ||r|| falls below it, leaving p for the resumed iteration.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, KVec<T> v, const double* part,
-                                                         KrylovState* st, double tol)
+                                                         KrylovState* st, double tol,
+                                                         double upd_thresh)
 {
     if (st->done) return;
     double s[2];
     reduce_partials<2>(part, gridDim.x, s);
-    const double error = sqrt(s[0]) / st->bnorm;
-    if (error < tol) {
+    const double rnorm = sqrt(s[0]);
+    const double error = rnorm / st->bnorm;
+    if (error < tol || rnorm < upd_thresh) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->error = error;
             st->iter = st->iter + 1;
-            st->done = 1;
+            st->done = error < tol ? 1 : 5;
         }
         return;
     }
     const T beta = (T)(s[1] / st->rr_old);
-    vec_chunk(n, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
+    {
+        T* __restrict__ p = v.p;
+        const T* __restrict__ r = v.r;
+        const T* __restrict__ d = v.d;
+        int beg, end;
+        chunk_of(n, beg, end);
+        constexpr int U = 4;
+        int i = beg + threadIdx.x;
+        for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
+            T pi[U], ri[U], di[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = i + u * kBlock;
+                pi[u] = p[k];
+                ri[u] = r[k];
+                di[u] = d[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
+        }
+        for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const int it = st->iter + 1;
         st->error = error;
         st->rr = s[1];
         st->iter = it;
         if (it >= st->max_iter) st->done = 2;
+    }
+}
+
+// Reliable update: after the FP64 residual replaced r, partials (r,r) and
+// (r/d, r) of the new r.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_rescan(int n, KVec<T> v, double* part)
+{
+    double acc[2] = {0.0, 0.0};
+    vec_chunk(n, [&](int i) {
+        const T ri = v.r[i];
+        const T zi = ri / v.d[i];
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)zi * (double)ri;
+    });
+    store_partials<2>(acc, part);
+}
+
+// ... then the direction step that the update interrupted:
+// beta = (z,r)_new / rr_old, p = z + beta p, and the recurrence resumes.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_resume(int n, KVec<T> v, const double* part,
+                                                      KrylovState* st)
+{
+    double s[2];
+    reduce_partials<2>(part, gridDim.x, s);
+    const T beta = (T)(s[1] / st->rr_old);
+    vec_chunk(n, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr = s[1];
+        st->done = 0;
     }
 }
 
@@ -895,6 +1148,15 @@ __global__ __launch_bounds__(kBlock) void k_fill(int64_t n, T* p, T v)
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock)
         p[i] = v;
+}
+
+// Plain streaming copy (bandwidth calibration): 16-byte loads and stores.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(int64_t n4, const double2* __restrict__ a,
+                                                        double2* __restrict__ b)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x * kBlock)
+        b[i] = a[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_cast_f32(int64_t n, const double* a, float* b)

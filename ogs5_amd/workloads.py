@@ -169,8 +169,8 @@ def make_system(sc: Scenario, precision: int = L.FP64):
 
 
 def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SYMMETRIC,
-            precision: int = L.FP64, inner_tol: float = 1e-4, system=None, keep_steps=False,
-            tol: Optional[float] = None) -> dict:
+            precision: int = L.FP64, inner_tol: Optional[float] = None, system=None,
+            keep_steps=False, tol: Optional[float] = None, ir_mode: Optional[int] = None) -> dict:
     """Runs every step of the scenario on the GPU; returns the final x, the
     per-step solve info and (keep_steps) every step's solution."""
     if system is None:
@@ -190,7 +190,7 @@ def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SY
             s.add_rhs(sc.st_idx, sc.st_val)
         s.set_dirichlet(sc.bc_idx, sc.bc_val, dirichlet)
         info = s.solve(method=method, tol=sc.tol if tol is None else tol,
-                       max_iter=sc.max_iter, inner_tol=inner_tol)
+                       max_iter=sc.max_iter, inner_tol=inner_tol, ir_mode=ir_mode)
         infos.append(info)
         s.copy_vector(L.VEC_UOLD, L.VEC_X)
         if keep_steps:
