@@ -114,6 +114,29 @@ def cpu_baseline(iters_per_solve, n_target, nnz_target, e_target, n_solves):
                 assembly_s_per_elem=per_elem, iter_s_per_nnz=per_nnz_iter)
 
 
+def spmv_mode(size):
+    """Short run for hardware-counter passes: a 1 GiB copy (calibration of
+    the byte counters) and 20 SpMV launches each of the FP64 and FP32 matrix
+    of the assembled steady 128^3 system."""
+    import ogs5_amd as ox
+    from ogs5_amd import workloads as W
+    sc = W.hex_flow(size, n_transient=0)
+    pat, s = W.make_system(sc, ox.FP32_IR)
+    st = sc.steps[0]
+    s.set_vector(ox.VEC_UOLD, sc.u0)
+    s.set_vector(ox.VEC_X, sc.u0)
+    s.initialize()
+    s.assemble(st.dt, st.theta, True)
+    s.set_dirichlet(sc.bc_idx, sc.bc_val, ox.DIRICHLET_SYMMETRIC)
+    copy = ox._lib.stream_copy_gbs(1 << 30, 5)
+    ms64, b64 = s.bench_spmv(False, 20)
+    ms32, b32 = s.bench_spmv(True, 20)
+    print(json.dumps({"copy_1GiB_gbs": copy, "spmv64": {"avg_ms": ms64, "bytes": b64,
+                                                          "gbs": b64 / ms64 / 1e6},
+                      "spmv32": {"avg_ms": ms32, "bytes": b32, "gbs": b32 / ms32 / 1e6},
+                      "variant64": s.spmv_variant(), "tuning": s.spmv_tuning()}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,11 +144,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["full", "spmv"], default="full")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "spmv":
+        spmv_mode(args.size)
+        return
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
 
@@ -145,6 +172,8 @@ def main():
         tr = json.load(open(tf)).get(roof["kernel"])
         if tr:
             roof["traffic"] = tr
+    copy_gbs = ox._lib.stream_copy_gbs(1 << 30, 10)
+    roof["copy_kernel_gbs"] = copy_gbs
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(fp64["iterations"], sc.n, _nnz(sc), sc.mesh.n_elem, len(sc.steps))

@@ -242,6 +242,12 @@ int ogsx_get_spmv_variant(ogsx_system* s, int* variant);
  * for IR systems, FP32 entries); *count = entries available. */
 int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* variants, double* ms,
                      int* count);
+/* Runs `reps` SpMV launches of the system's FP64 (fp32 = 0) or FP32 matrix
+ * with its tuned variant on device-resident vectors, timed with HIP events on
+ * the system's stream: average ms per launch and the launch's algorithmic
+ * bytes (stored matrix entries, column data, slice/row tables, x and y). */
+int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms,
+                    double* bytes_per_launch);
 /* Bandwidth calibration: a plain 16-byte-per-lane copy kernel over two
  * device buffers of `bytes` each, run `reps` times; *gbs = (read + write)
  * bytes / second / 1e9. */

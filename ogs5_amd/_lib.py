@@ -86,6 +86,8 @@ SIGNATURES = {
     "ogsx_set_spmv_variant": (C.c_int, [_P, C.c_int]),
     "ogsx_get_spmv_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "ogsx_spmv_tuning": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(C.c_int)]),
+    "ogsx_bench_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
     "ogsx_stream_copy": (C.c_int, [_I64, C.c_int, C.POINTER(C.c_double)]),
     "ogsx_synchronize": (C.c_int, [_P]),
 }
@@ -278,6 +280,12 @@ class System:
         v = C.c_int()
         check(lib().ogsx_get_spmv_variant(self.h, C.byref(v)), "ogsx_get_spmv_variant")
         return v.value
+
+    def bench_spmv(self, fp32=False, reps=20):
+        ms, b = C.c_double(), C.c_double()
+        check(lib().ogsx_bench_spmv(self.h, int(bool(fp32)), int(reps), C.byref(ms), C.byref(b)),
+              "ogsx_bench_spmv")
+        return ms.value, b.value
 
     def spmv_tuning(self):
         f = np.zeros(32, np.int32)

@@ -1291,6 +1291,45 @@ int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* formats, double* ms, 
     return OGSX_OK;
 }
 
+int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* bytes_per_launch)
+{
+    if (!s || reps < 1 || !avg_ms) return fail(OGSX_EINVAL, "bad argument");
+    if (fp32 && s->precision != OGSX_FP32_IR) return fail(OGSX_EINVAL, "system has no FP32 matrix");
+    OK_TRY(autotune(s));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    if (!fp32) {
+        OK_TRY(sync_sell(s, is_sell(s->fmt64), false));
+        const MatView<double> mv = mat_view(s, (const double*)s->A);
+        (void)hipEventRecord(e0, s->stream);
+        for (int r = 0; r < reps; r++)
+            FMT_DISPATCH(s->fmt64, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
+                                       mv, s->v64.x, s->v64.s)));
+        (void)hipEventRecord(e1, s->stream);
+    } else {
+        if (!is_sell(s->fmt32))
+            k_cast_f32<<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, s->A32);
+        OK_TRY(sync_sell(s, false, is_sell(s->fmt32)));
+        const MatView<float> mv = mat_view(s, (const float*)s->A32);
+        (void)hipEventRecord(e0, s->stream);
+        for (int r = 0; r < reps; r++)
+            FMT_DISPATCH(s->fmt32, (k_spmv<L, float, float><<<s->G, kBlock, 0, s->stream>>>(
+                                       mv, s->v32.x, s->v32.s)));
+        (void)hipEventRecord(e1, s->stream);
+    }
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(hipGetLastError());
+    *avg_ms = ms / reps;
+    if (bytes_per_launch)
+        *bytes_per_launch = fp32 ? spmv_bytes(s, s->fmt32, 4, 4) : spmv_bytes(s, s->fmt64, 8, 8);
+    return OGSX_OK;
+}
+
 int ogsx_stream_copy(int64_t bytes, int reps, double* gbs)
 {
     if (bytes < 16 || reps < 1 || !gbs) return fail(OGSX_EINVAL, "bad argument");
