@@ -76,6 +76,7 @@ extern "C" {
 
 typedef struct ogsx_pattern ogsx_pattern;
 typedef struct ogsx_system ogsx_system;
+typedef struct ogsx_group ogsx_group;
 
 const char* ogsx_last_error(void);
 /* Number of visible GPUs (0 on a machine without one).  No other call. */
@@ -212,6 +213,40 @@ int ogsx_spmv(ogsx_system* s, const double* x, double* y);
 /* Linear_EQS::Write_BIN (equation_class.cpp:409; matrix_class.cpp:1403):
  * int64 n; int64 ptr[n+1]; int64 col[nnz]; f64 A[nnz]; f64 b[n]. */
 int ogsx_write_bin(ogsx_system* s, const char* path);
+
+/* ---- groups: many systems solved together ----------------------------- */
+/* No counterpart in the reference, which solves one Linear_EQS at a time
+ * (CRFProcess::Execute, rf_pcs.cpp:4817); for callers that run many
+ * independent systems (ensembles, request streams).  A group launches each
+ * phase of the Krylov iteration once over the blocks of all its running
+ * systems; every system keeps its own grid, partial sums, state and
+ * parameters, so its arithmetic, iteration count and solution are
+ * bit-identical to ogsx_solve on that system alone.  Members share the
+ * group's precision; solves running at the same time share one method.
+ * Group work runs on the group's own HIP stream and is ordered after the
+ * work already queued on each started system's stream. */
+/* use_graphs = 1 replays the batches of iterations as captured HIP graphs. */
+int ogsx_group_create(int precision, int use_graphs, ogsx_group** out);
+/* Waits for the group's work and releases its members (they stay valid). */
+int ogsx_group_destroy(ogsx_group* g);
+/* A system belongs to at most one group; while it does, ogsx_solve on it
+ * fails (OGSX_ESTATE) and ogsx_system_destroy needs ogsx_group_remove first. */
+int ogsx_group_add(ogsx_group* g, ogsx_system* s);
+int ogsx_group_remove(ogsx_group* g, ogsx_system* s);  /* not while solving */
+/* Starts solves of `count` members with cfg while others may be running
+ * (continuous batching).  Each starts from its current x, as ogsx_solve. */
+int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
+                     const ogsx_solver_config* cfg);
+/* Runs up to max_iterations Krylov iterations of the running solves (fewer
+ * when all finish); *n_active = solves still running. */
+int ogsx_group_advance(ogsx_group* g, int max_iterations, int* n_active);
+/* Progress of a system's last solve: *active = 1 while it runs in a group;
+ * info as ogsx_solve reports it once finished. */
+int ogsx_solve_status(ogsx_system* s, int* active, ogsx_solve_info* info);
+/* Starts every member with cfg and runs them to completion; infos in the
+ * order the members were added (count = members). */
+int ogsx_group_solve(ogsx_group* g, const ogsx_solver_config* cfg,
+                     ogsx_solve_info* infos);
 
 /* ---- measurement ------------------------------------------------------ */
 /* Enables HIP-event timing of every SpMV launch that ogsx_solve issues (on

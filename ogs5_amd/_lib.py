@@ -90,6 +90,14 @@ SIGNATURES = {
                                   C.POINTER(C.c_double)]),
     "ogsx_stream_copy": (C.c_int, [_I64, C.c_int, C.POINTER(C.c_double)]),
     "ogsx_synchronize": (C.c_int, [_P]),
+    "ogsx_group_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P)]),
+    "ogsx_group_destroy": (C.c_int, [_P]),
+    "ogsx_group_add": (C.c_int, [_P, _P]),
+    "ogsx_group_remove": (C.c_int, [_P, _P]),
+    "ogsx_group_start": (C.c_int, [_P, C.POINTER(_P), C.c_int, C.POINTER(SolverConfig)]),
+    "ogsx_group_advance": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "ogsx_solve_status": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(SolveInfo)]),
+    "ogsx_group_solve": (C.c_int, [_P, C.POINTER(SolverConfig), C.POINTER(SolveInfo)]),
 }
 
 _lib = None
@@ -140,6 +148,17 @@ def _f64(a):
 
 def _i64(a):
     return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def solver_config(method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
+                  inner_tol=None, max_outer=50, ir_mode=None) -> SolverConfig:
+    """ogsx_solver_config; for IR systems CG defaults to reliable updates
+    (inner_tol = delta 0.03), BiCGStab to restarted refinement (1e-4)."""
+    if ir_mode is None:
+        ir_mode = IR_RELIABLE if method == CG else IR_RESTART
+    if inner_tol is None:
+        inner_tol = 0.03 if ir_mode == IR_RELIABLE else 1e-4
+    return SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer, ir_mode)
 
 
 class Pattern:
@@ -228,11 +247,7 @@ class System:
               inner_tol=None, max_outer=50, ir_mode=None) -> dict:
         """IR systems: CG defaults to reliable updates (inner_tol = delta 0.03),
         BiCGStab to restarted refinement (inner_tol 1e-4)."""
-        if ir_mode is None:
-            ir_mode = IR_RELIABLE if method == CG else IR_RESTART
-        if inner_tol is None:
-            inner_tol = 0.03 if ir_mode == IR_RELIABLE else 1e-4
-        cfg = SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer, ir_mode)
+        cfg = solver_config(method, precond, tol, max_iter, inner_tol, max_outer, ir_mode)
         info = SolveInfo()
         check(lib().ogsx_solve(self.h, C.byref(cfg), C.byref(info)), "ogsx_solve")
         return info.as_dict()
@@ -308,3 +323,62 @@ class System:
             self.close()
         except Exception:
             pass
+
+
+class Group:
+    """Systems solved together (ogsx_group_*): one launch per Krylov phase
+    over all running members; each member's results are those of
+    System.solve on it alone."""
+
+    def __init__(self, precision: int = FP64, use_graphs: bool = True):
+        h = C.c_void_p()
+        check(lib().ogsx_group_create(int(precision), int(bool(use_graphs)), C.byref(h)),
+              "ogsx_group_create")
+        self.h = h
+        self.precision = precision
+        self.members = []
+
+    def add(self, system: "System"):
+        check(lib().ogsx_group_add(self.h, system.h), "ogsx_group_add")
+        self.members.append(system)
+
+    def remove(self, system: "System"):
+        check(lib().ogsx_group_remove(self.h, system.h), "ogsx_group_remove")
+        self.members.remove(system)
+
+    def start(self, systems, **cfg):
+        arr = (C.c_void_p * len(systems))(*[s.h.value for s in systems])
+        c = solver_config(**cfg)
+        check(lib().ogsx_group_start(self.h, arr, len(systems), C.byref(c)), "ogsx_group_start")
+
+    def advance(self, max_iterations: int = 1 << 30) -> int:
+        n = C.c_int()
+        check(lib().ogsx_group_advance(self.h, int(max_iterations), C.byref(n)),
+              "ogsx_group_advance")
+        return n.value
+
+    def solve(self, **cfg):
+        infos = (SolveInfo * max(1, len(self.members)))()
+        c = solver_config(**cfg)
+        check(lib().ogsx_group_solve(self.h, C.byref(c), infos), "ogsx_group_solve")
+        return [infos[i].as_dict() for i in range(len(self.members))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ogsx_group_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_status(system: "System"):
+    """(active, info) of the system's last solve."""
+    a = C.c_int()
+    info = SolveInfo()
+    check(lib().ogsx_solve_status(system.h, C.byref(a), C.byref(info)), "ogsx_solve_status")
+    return bool(a.value), info.as_dict()

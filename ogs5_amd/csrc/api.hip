@@ -7,7 +7,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -62,6 +64,30 @@ struct ProfAcc {
     int64_t launches = 0;
 };
 
+namespace {
+struct MemberSet;
+
+enum : int { kPhaseKrylov = 0, kPhaseResidual = 1 };
+
+// Host-side progress of one system's solve (one member of a solve set).
+struct SolveCtx {
+    int active = 0;           // solve in progress
+    int method = OGSX_CG;
+    ogsx_solver_config cfg{};
+    bool ir = false;          // FP32 recurrence inside FP64 refinement
+    bool reliable = false;    // reliable-update CG (else restarted IR)
+    double delta = 0.0;       // inner tolerance / reliable-update threshold
+    int max_outer = 50;
+    int phase = kPhaseKrylov; // IR: waiting for the FP64 residual step
+    int inner_started = 0;    // IR: an FP32 recurrence has run
+    int inner_status = 0;     // IR: how the last recurrence stopped (1 conv/update, 2 max_iter, 3 breakdown)
+    int last_inner_iters = 0;
+    int total = 0, updates = 0;
+    int converged = 0, ret = 0;
+    double error = 0.0, bnorm = 0.0, maxr = 1.0;
+};
+}  // namespace
+
 struct ogsx_system {
     const PatternHost* pat = nullptr;
     int precision = OGSX_FP64;
@@ -100,8 +126,15 @@ struct ogsx_system {
     double* part = nullptr; // partial sums [4][G] x 3 sets
     double* red = nullptr;  // reduced scalars
     KrylovState* st = nullptr;
-    KrylovState* st_host = nullptr;  // pinned
-    double* red_host = nullptr;      // pinned
+    // device descriptor of this system and its per-solve parameters
+    SysDev devh{};
+    SysDev* dev = nullptr;
+    KrylovParams* prm = nullptr;
+    KrylovParams* prm_host = nullptr;  // pinned
+    SolveCtx sc;
+    MemberSet* solo = nullptr;         // set of one for ogsx_solve
+    ogsx_group* group = nullptr;       // group the system belongs to, if any
+    hipEvent_t ready = nullptr;        // marks the system's stream for the group stream
     // BC / ST
     int8_t* bcflag = nullptr;
     double* bcg = nullptr;
@@ -233,13 +266,13 @@ void build_sell(const PatternHost& h, std::vector<int>& sptr, std::vector<int>& 
 // Brings the SELL value copies up to date with A (after assembly / BC).
 int sync_sell(ogsx_system* s, bool want64, bool want32)
 {
-    const int nb = blocks_for(s->npad, kBlock * 4);
+    const Grp g1{s->dev, nullptr, nullptr};
     if (want64 && s->sval64_version != s->a_version) {
-        k_to_sell<double><<<nb, kBlock, 0, s->stream>>>(s->npad, s->sell_src, s->A, s->sval64);
+        k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1);
         s->sval64_version = s->a_version;
     }
     if (want32 && s->sval32 && s->sval32_version != s->a_version) {
-        k_to_sell<float><<<nb, kBlock, 0, s->stream>>>(s->npad, s->sell_src, s->A, s->sval32);
+        k_to_sell_g<float><<<s->G, kBlock, 0, s->stream>>>(g1);
         s->sval32_version = s->a_version;
     }
     HIP_TRY(hipGetLastError());
@@ -412,169 +445,6 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
     return nnz * (sval + sidx) + (n + 1) * 4 + 2 * n * sx + tiles;
 }
 
-template <typename VT, typename T>
-void launch_cg_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, double tol,
-                     int prof_kind, double upd_thresh)
-{
-    const int G = s->G;
-    const MatView<VT> mv = mat_view(s, val);
-    double* pA = s->part;
-    double* pB = s->part + 4 * G;
-    for (int it = 0; it < iters; it++) {
-        int pk = prof_begin(s, prof_kind);
-        FMT_DISPATCH(fmt_of<VT>(s), (k_cg_spmv<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                        mv, dv.kv(), pA, s->st)));
-        prof_end(s, pk);
-        k_cg_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pA, pB, s->st);
-        k_cg_direction<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pB, s->st, tol, upd_thresh);
-    }
-}
-
-template <typename VT, typename T>
-void launch_bi_chunk(ogsx_system* s, const VT* val, DevVecs<T>& dv, int iters, double tol,
-                     int prof_kind)
-{
-    const int G = s->G;
-    const MatView<VT> mv = mat_view(s, val);
-    double* pR = s->part;          // (r,r), (r0,r)
-    double* pV = s->part + 4 * G;  // (r0,v)
-    double* pS = s->part + 8 * G;  // (s,s)
-    double* pT = s->part + 12 * G; // (t,t), (t,s)
-    for (int it = 0; it < iters; it++) {
-        k_bi_dir<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pR, s->st, tol);
-        int pk = prof_begin(s, prof_kind);
-        FMT_DISPATCH(fmt_of<VT>(s), (k_bi_spmv_v<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                        mv, dv.kv(), pV, s->st)));
-        prof_end(s, pk);
-        k_bi_s<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pV, pS, s->st);
-        pk = prof_begin(s, prof_kind);
-        FMT_DISPATCH(fmt_of<VT>(s), (k_bi_spmv_t<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                        mv, dv.kv(), pS, pT, s->st, tol)));
-        prof_end(s, pk);
-        k_bi_update<T><<<G, kBlock, 0, s->stream>>>(s->n, dv.kv(), pT, pR, s->st);
-    }
-}
-
-// Launches chunks of iterations until the state leaves "running"; the chunk
-// grows from 4 to 64 iterations between host checks.
-template <typename VT, typename T>
-int krylov_loop(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol,
-                int max_iter, int prof_kind, double upd_thresh, int* launched)
-{
-    int chunk = 4;
-    while (!s->st_host->done) {
-        const int c = std::min(chunk, max_iter + 1 - *launched);
-        if (c <= 0) break;
-        if (method == OGSX_CG)
-            launch_cg_chunk<VT, T>(s, val, dv, c, tol, prof_kind, upd_thresh);
-        else
-            launch_bi_chunk<VT, T>(s, val, dv, c, tol, prof_kind);
-        *launched += c;
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
-                               s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        OK_TRY(prof_flush(s));
-        chunk = std::min(chunk * 2, 64);
-    }
-    return OGSX_OK;
-}
-
-// Runs one Krylov solve on (val, dv) from dv.x; fills st_host.
-template <typename VT, typename T>
-int krylov(ogsx_system* s, const VT* val, DevVecs<T>& dv, int method, double tol, int max_iter,
-           double bnorm_prev, int prof_kind)
-{
-    const int G = s->G;
-    const MatView<VT> mv = mat_view(s, val);
-    if (method == OGSX_CG) {
-        FMT_DISPATCH(fmt_of<VT>(s), (k_cg_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                        mv, dv.kv(), s->part)));
-        k_cg_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, bnorm_prev, tol, max_iter);
-    } else {
-        FMT_DISPATCH(fmt_of<VT>(s), (k_bi_init<L, VT, T><<<G, kBlock, 0, s->stream>>>(
-                                        mv, dv.kv(), s->part)));
-        k_bi_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, bnorm_prev, tol, max_iter);
-    }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost,
-                           s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    int launched = 0;
-    return krylov_loop<VT, T>(s, val, dv, method, tol, max_iter, prof_kind, 0.0, &launched);
-}
-
-// FP64 residual r64 = b - A x64 and its norms (host copies in red_host[0..1]).
-int ir_residual(ogsx_system* s)
-{
-    const int G = s->G;
-    FMT_DISPATCH(s->fmt64, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
-                               mat_view(s, (const double*)s->A), s->v64.x, s->v64.b, s->r64,
-                               s->part)));
-    k_reduce_to<<<1, kBlock, 0, s->stream>>>(G, s->part, 2, s->red);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(s->red_host, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    return OGSX_OK;
-}
-
-// Mixed-precision CG with reliable updates: the Jacobi-CG recurrence runs in
-// FP32 (matrix, vectors, SpMV, preconditioner) on r scaled by sigma =
-// 1/||r0||; whenever the FP32 residual falls by delta since the last update
-// (or claims convergence), the FP32 iterate is folded into the FP64 x, the
-// true residual b - A x is recomputed in FP64, and the recurrence resumes
-// from it with the same search direction (Clark et al., Comput. Phys.
-// Commun. 181 (2010) 1517, "reliable updates").  The stopping test is the
-// FP64 residual.
-int reliable_cg(ogsx_system* s, const ogsx_solver_config* cfg, double delta, double bnorm,
-                double rnorm0, int* total, int* updates, double* error, int* converged)
-{
-    const int G = s->G;
-    const double sigma = 1.0 / rnorm0;
-    const double tol_in = cfg->tol * bnorm * sigma;  // FP64 target in scaled units
-    double maxr = 1.0;
-    int launched = 0;
-    k_ir_prepare<<<G, kBlock, 0, s->stream>>>(s->n, s->r64, sigma, s->v32.b, s->v32.x);
-    const MatView<float> mv = mat_view(s, (const float*)s->A32);
-    FMT_DISPATCH(s->fmt32, (k_cg_init<L, float, float><<<G, kBlock, 0, s->stream>>>(
-                               mv, s->v32.kv(), s->part)));
-    k_cg_init_final<<<1, kBlock, 0, s->stream>>>(G, s->part, s->st, 0.0, tol_in, cfg->max_iter);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(s->st_host, s->st, sizeof(KrylovState), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    for (;;) {
-        OK_TRY(krylov_loop<float, float>(s, s->A32, s->v32, OGSX_CG, tol_in, cfg->max_iter,
-                                         OGSX_PROF_SPMV32, delta * maxr, &launched));
-        const KrylovState st = *s->st_host;
-        *total = st.iter;
-        // fold the FP32 iterate into x and recompute the FP64 residual
-        k_ir_correct<<<G, kBlock, 0, s->stream>>>(s->n, s->v64.x, s->v32.x, rnorm0);
-        OK_TRY(ir_residual(s));
-        (*updates)++;
-        const double rn = std::sqrt(s->red_host[1]);
-        *error = rn / bnorm;
-        if (*error < cfg->tol) {
-            *converged = 1;
-            return OGSX_OK;
-        }
-        if (st.done == 2 || st.iter >= cfg->max_iter) {
-            *converged = 0;
-            return OGSX_OK;
-        }
-        if (st.done == 3) {
-            *converged = 3;
-            return OGSX_OK;
-        }
-        // resume: r32 = sigma r64, x32 = 0, then the interrupted direction step
-        k_ir_prepare<<<G, kBlock, 0, s->stream>>>(s->n, s->r64, sigma, s->v32.r, s->v32.x);
-        k_cg_rescan<float><<<G, kBlock, 0, s->stream>>>(s->n, s->v32.kv(), s->part);
-        k_cg_resume<float><<<G, kBlock, 0, s->stream>>>(s->n, s->v32.kv(), s->part, s->st);
-        HIP_TRY(hipGetLastError());
-        maxr = rn * sigma;
-        s->st_host->done = 0;
-    }
-}
-
 template <typename T>
 int alloc_vecs(ogsx_system* s, DevVecs<T>& v)
 {
@@ -584,6 +454,8 @@ int alloc_vecs(ogsx_system* s, DevVecs<T>& v)
 }
 
 }  // namespace
+
+#include "solver.inc"
 
 // =========================================================================
 // C ABI
@@ -654,10 +526,12 @@ int ogsx_pattern_destroy(ogsx_pattern* p)
 int ogsx_system_destroy(ogsx_system* s)
 {
     if (!s) return OGSX_OK;
+    if (s->group) return fail(OGSX_ESTATE, "system belongs to a group: destroy the group first");
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    delete s->solo;
+    if (s->ready) (void)hipEventDestroy(s->ready);
     for (void* q : s->allocs) (void)hipFree(q);
-    if (s->st_host) (void)hipHostFree(s->st_host);
-    if (s->red_host) (void)hipHostFree(s->red_host);
+    if (s->prm_host) (void)hipHostFree(s->prm_host);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -706,9 +580,10 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
             (rc = dalloc(s, &s->r64, h.n)))
             return cleanup(rc);
     }
-    if (hipHostMalloc((void**)&s->st_host, sizeof(KrylovState)) != hipSuccess ||
-        hipHostMalloc((void**)&s->red_host, 8 * sizeof(double)) != hipSuccess)
+    if ((rc = dalloc(s, &s->dev, 1)) || (rc = dalloc(s, &s->prm, 1))) return cleanup(rc);
+    if (hipHostMalloc((void**)&s->prm_host, sizeof(KrylovParams)) != hipSuccess)
         return cleanup(fail(OGSX_ENOMEM, "hipHostMalloc failed"));
+    std::memset(s->prm_host, 0, sizeof(KrylovParams));
     auto up = [&](void* d, const void* src, size_t bytes) {
         return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s->stream);
     };
@@ -763,6 +638,35 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
             hipStreamSynchronize(s->stream))
             return cleanup(fail(OGSX_EHIP, "upload of the SELL layout failed"));
     }
+    {
+        SysDev& d = s->devh;
+        d.m64 = mat_view(s, (const double*)s->A);
+        d.m32 = mat_view(s, (const float*)s->A32);
+        d.v64 = s->v64.kv();
+        d.v32 = s->v32.kv();
+        d.A = s->A;
+        d.A32 = s->A32;
+        d.sval64 = s->sval64;
+        d.sval32 = s->sval32;
+        d.diagpos = s->diagpos;
+        d.sell_src = s->sell_src;
+        d.r64 = s->r64;
+        d.part = s->part;
+        d.red = s->red;
+        d.st = s->st;
+        d.prm = s->prm;
+        d.n = s->n;
+        d.G = s->G;
+        d.nnz = (int)s->nnz;
+        d.npad = (int)s->npad;
+        // no solve yet: the state reads "stopped" so group launches skip it
+        KrylovState st0;
+        std::memset(&st0, 0, sizeof(st0));
+        st0.done = 4;
+        if (up(s->dev, &d, sizeof(SysDev)) || up(s->st, &st0, sizeof(st0)) ||
+            up(s->prm, s->prm_host, sizeof(KrylovParams)) || hipStreamSynchronize(s->stream))
+            return cleanup(fail(OGSX_EHIP, "upload of the system descriptor failed"));
+    }
     *out = s;
     return OGSX_OK;
 }
@@ -770,6 +674,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
 int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e)
 {
     if (!s || !e) return fail(OGSX_EINVAL, "null argument");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     const PatternHost& h = *s->pat;
     const int ed = edim_of(h.etype);
     if (e->process != OGSX_LIQUID_FLOW && e->process != OGSX_HEAT_TRANSPORT)
@@ -890,6 +795,7 @@ extern "C" {
 int ogsx_system_initialize(ogsx_system* s)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     k_fill<double><<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, 0.0);
     k_fill<double><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v64.b, 0.0);
     s->a_version++;
@@ -900,6 +806,7 @@ int ogsx_system_initialize(ogsx_system* s)
 int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (!s->elements_set) return fail(OGSX_ESTATE, "ogsx_system_set_elements not called");
     if (!(dt > 0.0)) return fail(OGSX_EINVAL, "dt must be positive");
     AsmArgs a;
@@ -946,6 +853,7 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
 int ogsx_add_rhs(ogsx_system* s, int64_t n, const int64_t* idx, const double* val)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (n < 0 || (n > 0 && (!idx || !val))) return fail(OGSX_EINVAL, "bad source-term list");
     if (n == 0) return OGSX_OK;
     for (int64_t k = 0; k < n; k++)
@@ -994,6 +902,7 @@ int ogsx_add_rhs(ogsx_system* s, int64_t n, const int64_t* idx, const double* va
 int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx, const double* val, int mode)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (mode != OGSX_DIRICHLET_ROW && mode != OGSX_DIRICHLET_SYMMETRIC)
         return fail(OGSX_EINVAL, "unknown Dirichlet mode");
     if (n < 0 || (n > 0 && (!idx || !val))) return fail(OGSX_EINVAL, "bad Dirichlet list");
@@ -1041,117 +950,166 @@ int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx, const doub
 int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* info)
 {
     if (!s || !cfg) return fail(OGSX_EINVAL, "null argument");
-    if (cfg->method != OGSX_CG && cfg->method != OGSX_BICGSTAB)
-        return fail(OGSX_EINVAL, "method must be OGSX_CG (5) or OGSX_BICGSTAB (2)");
-    int precond = cfg->precond == 100 ? OGSX_PRECOND_JACOBI : cfg->precond;
-    if (precond != OGSX_PRECOND_JACOBI && precond != OGSX_PRECOND_NONE)
-        return fail(OGSX_EINVAL, "unknown preconditioner");
-    if (cfg->max_iter < 0 || !(cfg->tol > 0.0)) return fail(OGSX_EINVAL, "bad tolerance or max_iter");
-    ogsx_solve_info out;
-    std::memset(&out, 0, sizeof(out));
-    const int jac = precond == OGSX_PRECOND_JACOBI;
-    const int nb = blocks_for(s->n, kBlock);
-    OK_TRY(autotune(s));
-    OK_TRY(sync_sell(s, is_sell(s->fmt64), s->precision == OGSX_FP32_IR && is_sell(s->fmt32)));
-    if (s->precision == OGSX_FP64) {
-        k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
-        OK_TRY(krylov<double, double>(s, s->A, s->v64, cfg->method, cfg->tol, cfg->max_iter,
-                                      s->bnorm, OGSX_PROF_SPMV64));
-        const KrylovState& st = *s->st_host;
-        out.iterations = st.iter;
-        out.error = st.error;
-        out.bnorm = st.bnorm;
-        out.converged = st.done == 1 ? 1 : st.done == 2 ? 0 : st.done == 4 ? 2 : 3;
-        if (cfg->method == OGSX_BICGSTAB)
-            out.ret = st.done == 2 ? cfg->max_iter + 1 : (st.done == 3 && st.iter == 0) ? 0 : st.iter;
-        else
-            out.ret = st.done == 4 ? 0 : (st.done == 2 ? 0 : 1);
-        if (st.done == 3 && cfg->method == OGSX_BICGSTAB && st.rho1 == 0.0) out.ret = 0;
-        s->bnorm = st.bnorm;
-    } else {
-        // FP32 inner Krylov inside FP64 iterative refinement
-        const int G = s->G;
-        k_diag<double><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v64.d, jac);
-        k_diag<float><<<nb, kBlock, 0, s->stream>>>(s->n, s->diagpos, s->A, s->v32.d, jac);
-        if (!is_sell(s->fmt32) && s->a32_version != s->a_version) {
-            k_cast_f32<<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, s->A32);
-            s->a32_version = s->a_version;
-        }
-        const double inner_tol = cfg->inner_tol > 0.0 ? cfg->inner_tol : 1e-4;
-        const int max_outer = cfg->max_outer > 0 ? cfg->max_outer : 50;
-        double bnorm_new = 0.0;
-        int total = 0, outer = 0;
-        int converged = 0;
-        double error = 0.0;
-        const bool reliable = cfg->method == OGSX_CG && cfg->ir_mode == OGSX_IR_RELIABLE;
-        for (;;) {
-            FMT_DISPATCH(s->fmt64, (k_ir_residual<L><<<G, kBlock, 0, s->stream>>>(
-                                       mat_view(s, (const double*)s->A), s->v64.x, s->v64.b,
-                                       s->r64, s->part)));
-            k_reduce_to<<<1, kBlock, 0, s->stream>>>(G, s->part, 2, s->red);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(s->red_host, s->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-            HIP_TRY(hipStreamSynchronize(s->stream));
-            const double bb = s->red_host[0], rr = s->red_host[1];
-            if (outer == 0) {
-                bnorm_new = std::sqrt(bb);
-                // CheckNormRHS (equation_class.cpp:1104)
-                if (s->bnorm > 0.0 && bnorm_new / s->bnorm < cfg->tol) {
-                    error = bnorm_new / s->bnorm;
-                    converged = 2;
-                    break;
-                }
-                if (bnorm_new < DBL_MIN) {
-                    error = 0.0;
-                    converged = 2;
-                    break;
-                }
-            }
-            const double rnorm = std::sqrt(rr);
-            error = rnorm / bnorm_new;
-            if (error < cfg->tol) {
-                converged = 1;
-                break;
-            }
-            if (reliable) {
-                const double delta = cfg->inner_tol > 0.0 ? cfg->inner_tol : 0.03;
-                OK_TRY(reliable_cg(s, cfg, delta, bnorm_new, rnorm, &total, &outer, &error,
-                                   &converged));
-                break;
-            }
-            if (outer >= max_outer || total >= cfg->max_iter) {
-                converged = 0;
-                break;
-            }
-            k_ir_prepare<<<G, kBlock, 0, s->stream>>>(s->n, s->r64, 1.0 / rnorm, s->v32.b, s->v32.x);
-            const int budget = std::max(1, cfg->max_iter - total);
-            OK_TRY(krylov<float, float>(s, s->A32, s->v32, cfg->method, inner_tol, budget, 0.0,
-                                        OGSX_PROF_SPMV32));
-            const KrylovState& st = *s->st_host;
-            total += std::max(st.iter, 1);
-            if (st.done == 3 && st.iter == 0) {
-                converged = 3;
-                break;
-            }
-            k_ir_correct<<<G, kBlock, 0, s->stream>>>(s->n, s->v64.x, s->v32.x, rnorm);
-            outer++;
-        }
-        out.iterations = total;
-        out.outer = outer;
-        out.error = error;
-        out.bnorm = bnorm_new;
-        out.converged = converged;
-        out.ret = converged == 1 ? total : 0;
-        s->bnorm = bnorm_new;
-    }
+    OK_TRY(check_config(cfg));
+    if (s->group) return fail(OGSX_ESTATE, "system belongs to a group: solve it through the group");
+    if (!s->solo) OK_TRY(make_set({s}, s->stream, &s->solo));
+    MemberSet* m = s->solo;
+    OK_TRY(prepare_member(s, cfg, s->stream));
+    OK_TRY(set_full_maps(m));
+    OK_TRY(set_start(m, {0}));
+    OK_TRY(set_advance(m, INT_MAX));
     HIP_TRY(hipGetLastError());
-    if (info) *info = out;
+    if (info) info_of(s, info);
+    return OGSX_OK;
+}
+
+}  // extern "C"
+
+struct ogsx_group {
+    int precision = OGSX_FP64;
+    hipStream_t stream = nullptr;
+    MemberSet* set = nullptr;
+};
+
+namespace {
+int member_index(const ogsx_group* g, const ogsx_system* s)
+{
+    const auto& mem = g->set->mem;
+    for (size_t i = 0; i < mem.size(); i++)
+        if (mem[i] == s) return (int)i;
+    return -1;
+}
+}  // namespace
+
+extern "C" {
+
+int ogsx_group_create(int precision, int use_graphs, ogsx_group** out)
+{
+    if (!out) return fail(OGSX_EINVAL, "null output handle");
+    *out = nullptr;
+    if (precision != OGSX_FP64 && precision != OGSX_FP32_IR) return fail(OGSX_EINVAL, "unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(OGSX_EHIP, "no HIP device available");
+    auto* g = new (std::nothrow) ogsx_group();
+    if (!g) return fail(OGSX_ENOMEM, "host allocation failed");
+    g->precision = precision;
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return fail(OGSX_EHIP, "hipStreamCreate failed");
+    }
+    const int rc = make_set({}, g->stream, &g->set);
+    if (rc != OGSX_OK) {
+        (void)hipStreamDestroy(g->stream);
+        delete g;
+        return rc;
+    }
+    g->set->use_graphs = use_graphs != 0;
+    *out = g;
+    return OGSX_OK;
+}
+
+int ogsx_group_destroy(ogsx_group* g)
+{
+    if (!g) return OGSX_OK;
+    (void)hipStreamSynchronize(g->stream);
+    for (ogsx_system* s : g->set->mem) {
+        s->group = nullptr;
+        s->sc.active = 0;
+    }
+    delete g->set;
+    (void)hipStreamDestroy(g->stream);
+    delete g;
+    return OGSX_OK;
+}
+
+int ogsx_group_add(ogsx_group* g, ogsx_system* s)
+{
+    if (!g || !s) return fail(OGSX_EINVAL, "null argument");
+    if (s->group) return fail(OGSX_ESTATE, "system already belongs to a group");
+    if (s->precision != g->precision) return fail(OGSX_EINVAL, "system precision differs from the group's");
+    std::vector<ogsx_system*> mem = g->set->mem;
+    mem.push_back(s);
+    OK_TRY(set_members(g->set, mem));
+    s->group = g;
+    s->sc.active = 0;
+    return OGSX_OK;
+}
+
+int ogsx_group_remove(ogsx_group* g, ogsx_system* s)
+{
+    if (!g || !s) return fail(OGSX_EINVAL, "null argument");
+    const int i = member_index(g, s);
+    if (i < 0) return fail(OGSX_EINVAL, "system is not a member of this group");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    std::vector<ogsx_system*> mem = g->set->mem;
+    mem.erase(mem.begin() + i);
+    OK_TRY(set_members(g->set, mem));
+    s->group = nullptr;
+    return OGSX_OK;
+}
+
+int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
+                     const ogsx_solver_config* cfg)
+{
+    if (!g || !cfg || count < 0 || (count > 0 && !systems)) return fail(OGSX_EINVAL, "bad argument");
+    OK_TRY(check_config(cfg));
+    MemberSet* m = g->set;
+    for (ogsx_system* s : m->mem)
+        if (s->sc.active && s->sc.method != cfg->method)
+            return fail(OGSX_EINVAL, "solves running together in a group must share the method");
+    std::vector<int> which;
+    for (int k = 0; k < count; k++) {
+        const int i = member_index(g, systems[k]);
+        if (i < 0) return fail(OGSX_EINVAL, "system is not a member of this group");
+        if (systems[k]->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
+        if (std::find(which.begin(), which.end(), i) != which.end())
+            return fail(OGSX_EINVAL, "system listed twice");
+        which.push_back(i);
+    }
+    for (int i : which) {
+        ogsx_system* s = m->mem[i];
+        OK_TRY(prepare_member(s, cfg, s->stream));
+        if (!s->ready) HIP_TRY(hipEventCreateWithFlags(&s->ready, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(s->ready, s->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, s->ready, 0));
+    }
+    OK_TRY(set_full_maps(m));
+    return set_start(m, which);
+}
+
+int ogsx_group_advance(ogsx_group* g, int max_iterations, int* n_active)
+{
+    if (!g || max_iterations < 0) return fail(OGSX_EINVAL, "bad argument");
+    OK_TRY(set_advance(g->set, max_iterations, n_active));
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_solve_status(ogsx_system* s, int* active, ogsx_solve_info* info)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (active) *active = s->sc.active;
+    if (info) info_of(s, info);
+    return OGSX_OK;
+}
+
+int ogsx_group_solve(ogsx_group* g, const ogsx_solver_config* cfg, ogsx_solve_info* infos)
+{
+    if (!g || !cfg) return fail(OGSX_EINVAL, "null argument");
+    const std::vector<ogsx_system*> mem = g->set->mem;
+    OK_TRY(ogsx_group_start(g, mem.data(), (int)mem.size(), cfg));
+    int active = 1;
+    while (active) OK_TRY(set_advance(g->set, INT_MAX, &active));
+    if (infos)
+        for (size_t i = 0; i < mem.size(); i++) info_of(mem[i], &infos[i]);
     return OGSX_OK;
 }
 
 int ogsx_set_vector(ogsx_system* s, int which, const double* host)
 {
     if (!s || !host) return fail(OGSX_EINVAL, "null argument");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     double* d = nullptr;
     OK_TRY(check_vec(s, which, &d));
     HIP_TRY(hipMemcpyAsync(d, host, 8 * (size_t)s->n, hipMemcpyHostToDevice, s->stream));
@@ -1172,6 +1130,7 @@ int ogsx_get_vector(ogsx_system* s, int which, double* host)
 int ogsx_copy_vector(ogsx_system* s, int dst, int src)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     double *d = nullptr, *q = nullptr;
     OK_TRY(check_vec(s, dst, &d));
     OK_TRY(check_vec(s, src, &q));
@@ -1191,6 +1150,7 @@ int ogsx_get_matrix(ogsx_system* s, double* values)
 int ogsx_spmv(ogsx_system* s, const double* x, double* y)
 {
     if (!s || !x || !y) return fail(OGSX_EINVAL, "null argument");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     // the Krylov work vectors r and s serve as scratch
     HIP_TRY(hipMemcpyAsync(s->v64.r, x, 8 * (size_t)s->n, hipMemcpyHostToDevice, s->stream));
     int pk = prof_begin(s, OGSX_PROF_SPMV64);
@@ -1258,6 +1218,7 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms, int64_t* launc
 int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
 {
     if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (lanes_per_row == 0) {
         s->fmt64 = s->fmt32 = 0;  // autotune on next use
         return OGSX_OK;
@@ -1294,6 +1255,7 @@ int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* formats, double* ms, 
 int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* bytes_per_launch)
 {
     if (!s || reps < 1 || !avg_ms) return fail(OGSX_EINVAL, "bad argument");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (fp32 && s->precision != OGSX_FP32_IR) return fail(OGSX_EINVAL, "system has no FP32 matrix");
     OK_TRY(autotune(s));
     hipEvent_t e0, e1;
@@ -1309,7 +1271,7 @@ int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* 
         (void)hipEventRecord(e1, s->stream);
     } else {
         if (!is_sell(s->fmt32))
-            k_cast_f32<<<blocks_for(s->nnz, kBlock * 4), kBlock, 0, s->stream>>>(s->nnz, s->A, s->A32);
+            k_cast_f32_g<<<s->G, kBlock, 0, s->stream>>>(Grp{s->dev, nullptr, nullptr});
         OK_TRY(sync_sell(s, false, is_sell(s->fmt32)));
         const MatView<float> mv = mat_view(s, (const float*)s->A32);
         (void)hipEventRecord(e0, s->stream);

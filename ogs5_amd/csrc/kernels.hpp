@@ -26,9 +26,11 @@ namespace ogsx {
 // ------------------------------------------------------------------------
 // generic helpers
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void chunk_of(int n, int& beg, int& end)
+// Block b of G (G a multiple of 8) works on chunk (b % 8) * (G / 8) + b / 8
+// of n items.  In a group launch, b and G are the block's index and count
+// within its own system.
+__device__ __forceinline__ void chunk_of(int n, int G, int b, int& beg, int& end)
 {
-    const int G = gridDim.x, b = blockIdx.x;
     const int c = (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
     beg = (int)((int64_t)n * c / G);
     end = (int)((int64_t)n * (c + 1) / G);
@@ -43,7 +45,7 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // Block-reduces NV values and stores them as partials[k*G + blockIdx.x].
 template <int NV>
-__device__ __forceinline__ void store_partials(double (&v)[NV], double* partials)
+__device__ __forceinline__ void store_partials(double (&v)[NV], double* partials, int G, int b)
 {
     __shared__ double sm[NV][kBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -59,7 +61,7 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* partials
             double s = sm[k][0];
 #pragma unroll
             for (int w = 1; w < kBlock / 64; w++) s += sm[k][w];
-            partials[(size_t)k * gridDim.x + blockIdx.x] = s;
+            partials[(size_t)k * G + b] = s;
         }
     }
 }
@@ -114,11 +116,11 @@ __device__ __forceinline__ ACC row_dot(const int* __restrict__ ptr, const int* _
 template <int LPR, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_chunk(int n, const int* __restrict__ ptr,
                                            const int* __restrict__ col, const VT* __restrict__ val,
-                                           const XT* __restrict__ x, Epi&& epi)
+                                           const XT* __restrict__ x, int G, int b, Epi&& epi)
 {
     constexpr int RPB = kBlock / LPR;
     int beg, end;
-    chunk_of(n, beg, end);
+    chunk_of(n, G, b, beg, end);
     const int lane = threadIdx.x % LPR;
     for (int r = beg + threadIdx.x / LPR; r < end; r += RPB) {
         ACC s = row_dot<LPR, VT, XT, ACC>(ptr, col, val, x, r, lane);
@@ -127,10 +129,10 @@ __device__ __forceinline__ void spmv_chunk(int n, const int* __restrict__ ptr,
 }
 
 template <typename Body>
-__device__ __forceinline__ void vec_chunk(int n, Body&& body)
+__device__ __forceinline__ void vec_chunk(int n, int G, int b, Body&& body)
 {
     int beg, end;
-    chunk_of(n, beg, end);
+    chunk_of(n, G, b, beg, end);
     for (int i = beg + threadIdx.x; i < end; i += kBlock) body(i);
 }
 
@@ -183,12 +185,12 @@ constexpr int kSellC = 64;                // rows per slice = wavefront size
 // A tile whose single row exceeds kTileNnz is summed by the whole block.
 template <int C16, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __restrict__ x,
-                                           Epi&& epi)
+                                           int G, int b, Epi&& epi)
 {
 #pragma clang fp contract(off)
     __shared__ ACC prod[kTileNnz];
     int tb, te;
-    chunk_of(m.n_tiles, tb, te);
+    chunk_of(m.n_tiles, G, b, tb, te);
     const int tid = threadIdx.x;
     for (int t = tb; t < te; t++) {
         const int r0 = m.tile_row[t], r1 = m.tile_row[t + 1];
@@ -263,12 +265,12 @@ __device__ __forceinline__ T ld(const T* p, bool nt)
 }
 
 template <int C16, int U, bool NT, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_sell(const MatView<VT>& m, const XT* __restrict__ x,
-                                          Epi&& epi)
+__device__ __forceinline__ void spmv_sell(const MatView<VT>& m, const XT* __restrict__ x, int G,
+                                          int b, Epi&& epi)
 {
 #pragma clang fp contract(off)
     int sb, se;
-    chunk_of(m.n_slices, sb, se);
+    chunk_of(m.n_slices, G, b, sb, se);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int sl = sb + wave; sl < se; sl += kBlock / 64) {
         const int base = m.sell_ptr[sl] + lane;
@@ -309,11 +311,11 @@ __device__ __forceinline__ void spmv_sell(const MatView<VT>& m, const XT* __rest
 // flight; each row is still summed in CRS order.
 template <int U, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_sell_x2(const MatView<VT>& m, const XT* __restrict__ x,
-                                             Epi&& epi)
+                                             int G, int b, Epi&& epi)
 {
 #pragma clang fp contract(off)
     int sb, se;
-    chunk_of(m.n_slices, sb, se);
+    chunk_of(m.n_slices, G, b, sb, se);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int W = kBlock / 64;
     for (int sl = sb + wave; sl < se; sl += 2 * W) {
@@ -369,40 +371,27 @@ __device__ __forceinline__ void spmv_sell_x2(const MatView<VT>& m, const XT* __r
 // Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled or
 // SELL-64 format.
 template <int FMT, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_rows(const MatView<VT>& m, const XT* __restrict__ x,
-                                          Epi&& epi)
+__device__ __forceinline__ void spmv_rows(const MatView<VT>& m, const XT* __restrict__ x, int G,
+                                          int b, Epi&& epi)
 {
     if constexpr (FMT == kFmtTile32)
-        spmv_tiles<0, VT, XT, ACC>(m, x, epi);
+        spmv_tiles<0, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtTile16)
-        spmv_tiles<1, VT, XT, ACC>(m, x, epi);
+        spmv_tiles<1, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell32)
-        spmv_sell<0, 4, false, VT, XT, ACC>(m, x, epi);
+        spmv_sell<0, 4, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16)
-        spmv_sell<1, 4, false, VT, XT, ACC>(m, x, epi);
+        spmv_sell<1, 4, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16U8)
-        spmv_sell<1, 8, false, VT, XT, ACC>(m, x, epi);
+        spmv_sell<1, 8, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16NT)
-        spmv_sell<1, 8, true, VT, XT, ACC>(m, x, epi);
+        spmv_sell<1, 8, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16U16)
-        spmv_sell<1, 16, true, VT, XT, ACC>(m, x, epi);
+        spmv_sell<1, 16, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16X2)
-        spmv_sell_x2<4, VT, XT, ACC>(m, x, epi);
+        spmv_sell_x2<4, VT, XT, ACC>(m, x, G, b, epi);
     else
-        spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, epi);
-}
-
-// CRS values -> SELL-64 values (padding 0): sval[p] = A[src[p]].
-template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_to_sell(int64_t npad, const int* __restrict__ src,
-                                                    const double* __restrict__ A,
-                                                    VT* __restrict__ sval)
-{
-    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < npad;
-         p += (int64_t)gridDim.x * kBlock) {
-        const int k = src[p];
-        sval[p] = k >= 0 ? (VT)A[k] : (VT)0;
-    }
+        spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, G, b, epi);
 }
 
 // ------------------------------------------------------------------------
@@ -667,25 +656,28 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
 }
 
 // ------------------------------------------------------------------------
-// sparse matrix-vector product
+// sparse matrix-vector product (one system; autotuner, bench, ogsx_spmv)
 // ------------------------------------------------------------------------
 template <int FMT, typename VT, typename XT>
 __global__ __launch_bounds__(kBlock) void k_spmv(MatView<VT> m, const XT* __restrict__ x,
                                                  XT* __restrict__ y)
 {
     using ACC = XT;
-    spmv_rows<FMT, VT, XT, ACC>(m, x, [&](int r, ACC s) { y[r] = (XT)s; });
+    spmv_rows<FMT, VT, XT, ACC>(m, x, gridDim.x, blockIdx.x,
+                                [&](int r, ACC s) { y[r] = (XT)s; });
 }
 
 // ------------------------------------------------------------------------
-// Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
-// T = vector/arithmetic type, VT = matrix value type.
+// Krylov kernels.  Every kernel reads its system from a device descriptor
+// (SysDev); a group launch covers many systems, each with its own G blocks,
+// partial sums and state, so a system's arithmetic is the same alone and in
+// any mix of other systems.
 // ------------------------------------------------------------------------
 template <typename T>
 struct KVec {
     T* x;
-    const T* b;
-    const T* d;   // Jacobi diagonal (entries with |a_ii| < DBL_MIN are 1)
+    T* b;
+    T* d;   // Jacobi diagonal (entries with |a_ii| < DBL_MIN are 1)
     T* r;
     T* p;
     T* s;
@@ -697,12 +689,128 @@ struct KVec {
     T* sh;
 };
 
+// Per-solve parameters of one system, written by the host before a solve.
+struct KrylovParams {
+    double tol;         // stopping test ||r||/||b|| < tol
+    double bnorm_prev;  // CheckNormRHS reference (0 switches the test off)
+    double upd_thresh;  // reliable updates: stop the recurrence when ||r|| < it
+    double scale;       // IR: r32 = scale * r64
+    double unscale;     // IR: x64 += unscale * x32
+    int max_iter;
+    int jacobi;         // 1 Jacobi, 0 no preconditioner
+};
+
+struct SysDev {
+    MatView<double> m64;
+    MatView<float> m32;
+    KVec<double> v64;
+    KVec<float> v32;
+    double* A;           // CRS values (FP64)
+    float* A32;          // CRS values (FP32, IR)
+    double* sval64;      // SELL values
+    float* sval32;
+    const int* diagpos;
+    const int* sell_src;
+    double* r64;         // IR residual
+    double* part;        // partial sums [16][G]
+    double* red;         // reduced scalars
+    KrylovState* st;
+    const KrylovParams* prm;
+    int n, G, nnz, npad;
+};
+
+template <typename VT>
+__device__ __forceinline__ const MatView<VT>& mv_of(const SysDev& S);
+template <>
+__device__ __forceinline__ const MatView<double>& mv_of<double>(const SysDev& S) { return S.m64; }
+template <>
+__device__ __forceinline__ const MatView<float>& mv_of<float>(const SysDev& S) { return S.m32; }
+template <typename T>
+__device__ __forceinline__ const KVec<T>& kv_of(const SysDev& S);
+template <>
+__device__ __forceinline__ const KVec<double>& kv_of<double>(const SysDev& S) { return S.v64; }
+template <>
+__device__ __forceinline__ const KVec<float>& kv_of<float>(const SysDev& S) { return S.v32; }
+
+// Launch geometry: blk2sys = nullptr for one system (grid = its G blocks);
+// otherwise block -> index into sys, blk0[i] = first block of system i.
+// Kernels with one block per system use blk2sys as the block -> system list.
+struct Grp {
+    const SysDev* sys;
+    const int* blk2sys;
+    const int* blk0;
+};
+
+struct Ctx {
+    const SysDev* S;
+    int G, b;
+};
+
+__device__ __forceinline__ Ctx ctx_of(const Grp& g)
+{
+    if (!g.blk2sys) return Ctx{g.sys, (int)gridDim.x, (int)blockIdx.x};
+    const int si = g.blk2sys[blockIdx.x];
+    const SysDev* S = g.sys + si;
+    return Ctx{S, S->G, (int)blockIdx.x - g.blk0[si]};
+}
+
+__device__ __forceinline__ const SysDev* sys_of_block(const Grp& g)
+{
+    return g.sys + (g.blk2sys ? g.blk2sys[blockIdx.x] : (int)blockIdx.x);
+}
+
+// Jacobi diagonal: Precond_Jacobi's a_ii with |a_ii| < DBL_MIN -> 1
+// (FEM/matrix_class.cpp:1786-1821); no preconditioner -> 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    T* d = kv_of<T>(S).d;
+    const int jac = S.prm->jacobi;
+    vec_chunk(S.n, c.G, c.b, [&](int i) {
+        double v = (jac && S.diagpos[i] >= 0) ? S.A[S.diagpos[i]] : 1.0;
+        if (fabs(v) < DBL_MIN) v = 1.0;
+        d[i] = (T)v;
+    });
+}
+
+// CRS values -> SELL-64 values (padding 0) of one precision.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    VT* sv;
+    if constexpr (sizeof(VT) == 8)
+        sv = S.sval64;
+    else
+        sv = S.sval32;
+    vec_chunk(S.npad, c.G, c.b, [&](int p) {
+        const int k = S.sell_src[p];
+        sv[p] = k >= 0 ? (VT)S.A[k] : (VT)0;
+    });
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_f32_g(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    vec_chunk(S.nnz, c.G, c.b, [&](int k) { S.A32[k] = (float)S.A[k]; });
+}
+
+// ---- Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
+
 // s = A x; r = b - s; z = r/d; p = z; partials (b,b), (r,r), (r,z)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_init(MatView<VT> m, KVec<T> v, double* part)
+__global__ __launch_bounds__(kBlock) void k_cg_init(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const MatView<VT> m = mv_of<VT>(S);
+    const KVec<T> v = kv_of<T>(S);
     double acc[3] = {0.0, 0.0, 0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.x, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.x, c.G, c.b, [&](int i, T ax) {
         const T ri = v.b[i] - ax;
         const T zi = ri / v.d[i];
         v.r[i] = ri;
@@ -711,22 +819,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(MatView<VT> m, KVec<T> v, do
         acc[1] += (double)ri * (double)ri;
         acc[2] += (double)ri * (double)zi;
     });
-    store_partials<3>(acc, part);
+    store_partials<3>(acc, S.part, c.G, c.b);
 }
 
-// Single-block: CheckNormRHS (:1104), initial residual test, rr = (r,z).
-__global__ void k_cg_init_final(int G, const double* part, KrylovState* st, double bnorm_prev,
-                                double tol, int max_iter)
+// One block per system: CheckNormRHS (:1104), initial residual test, rr.
+__global__ void k_cg_init_final(Grp g)
 {
+    const SysDev& S = *sys_of_block(g);
     double s[3];
-    reduce_partials<3>(part, G, s);
+    reduce_partials<3>(S.part, S.G, s);
     if (threadIdx.x != 0) return;
+    KrylovState* st = S.st;
+    const KrylovParams P = *S.prm;
     const double bnorm_new = sqrt(s[0]);
     st->iter = 0;
-    st->max_iter = max_iter;
+    st->max_iter = P.max_iter;
     st->done = 0;
-    if (bnorm_prev > 0.0 && bnorm_new / bnorm_prev < tol) {
-        st->error = bnorm_new / bnorm_prev;
+    if (P.bnorm_prev > 0.0 && bnorm_new / P.bnorm_prev < P.tol) {
+        st->error = bnorm_new / P.bnorm_prev;
         st->bnorm = bnorm_new;
         st->done = 4;
         return;
@@ -738,7 +848,7 @@ __global__ void k_cg_init_final(int G, const double* part, KrylovState* st, doub
         return;
     }
     st->error = sqrt(s[1]) / bnorm_new;
-    if (st->error < tol) {
+    if (st->error < P.tol) {
         st->done = 1;
         return;
     }
@@ -748,36 +858,42 @@ __global__ void k_cg_init_final(int G, const double* part, KrylovState* st, doub
 
 // K1: s = A p; partial (p,s)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(MatView<VT> m, KVec<T> v, double* part,
-                                                    const KrylovState* st)
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
 {
-    if (st->done) return;
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    if (S.st->done) return;
+    const MatView<VT> m = mv_of<VT>(S);
+    const KVec<T> v = kv_of<T>(S);
     double acc[1] = {0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.p, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.p, c.G, c.b, [&](int i, T ax) {
         v.s[i] = ax;
         acc[0] += (double)v.p[i] * (double)ax;
     });
-    store_partials<1>(acc, part);
+    store_partials<1>(acc, S.part, c.G, c.b);
 }
 
 // K2: alpha = rr/(p,s); x += alpha p; r -= alpha s; partials (r,r), (r/d, r)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(int n, KVec<T> v, const double* part_ps,
-                                                      double* part, KrylovState* st)
+__global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
     double ps[1];
-    reduce_partials<1>(part_ps, gridDim.x, ps);
+    reduce_partials<1>(S.part, c.G, ps);
     const double rr = st->rr;
     const T alpha = (T)(rr / ps[0]);
     double acc[2] = {0.0, 0.0};
+    const KVec<T> v = kv_of<T>(S);
     T* __restrict__ x = v.x;
     T* __restrict__ r = v.r;
     const T* __restrict__ p = v.p;
-    const T* __restrict__ s = v.s;
+    const T* __restrict__ sv = v.s;
     const T* __restrict__ d = v.d;
     int beg, end;
-    chunk_of(n, beg, end);
+    chunk_of(S.n, c.G, c.b, beg, end);
     constexpr int U = 4;
     int i = beg + threadIdx.x;
     for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
@@ -788,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, KVec<T> v, const do
             xi[u] = x[k];
             pi[u] = p[k];
             ri[u] = r[k];
-            si[u] = s[k];
+            si[u] = sv[k];
             di[u] = d[k];
         }
 #pragma unroll
@@ -804,34 +920,36 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, KVec<T> v, const do
     }
     for (; i < end; i += kBlock) {
         x[i] += alpha * p[i];
-        const T rn = r[i] - alpha * s[i];
+        const T rn = r[i] - alpha * sv[i];
         r[i] = rn;
         const T zi = rn / d[i];
         acc[0] += (double)rn * (double)rn;
         acc[1] += (double)zi * (double)rn;
     }
-    store_partials<2>(acc, part);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, c.b);
+    if (c.b == 0 && threadIdx.x == 0) {
         st->rr_old = rr;
         st->alpha = rr / ps[0];
     }
 }
 
-// K3: error test; beta = (z,r)/rr_old; p = r/d + beta p.  upd_thresh > 0
-// (reliable-update mixed precision) stops the recurrence with done = 5 when
-// ||r|| falls below it, leaving p for the resumed iteration.
+// K3: error test; beta = (z,r)/rr_old; p = r/d + beta p.  With reliable
+// updates (upd_thresh > 0) the recurrence stops with done = 5 once ||r|| <
+// upd_thresh, leaving p for the resumed iteration.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_direction(int n, KVec<T> v, const double* part,
-                                                         KrylovState* st, double tol,
-                                                         double upd_thresh)
+__global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
     double s[2];
-    reduce_partials<2>(part, gridDim.x, s);
+    reduce_partials<2>(S.part + 4 * (size_t)c.G, c.G, s);
     const double rnorm = sqrt(s[0]);
     const double error = rnorm / st->bnorm;
-    if (error < tol || rnorm < upd_thresh) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double tol = S.prm->tol;
+    if (error < tol || rnorm < S.prm->upd_thresh) {
+        if (c.b == 0 && threadIdx.x == 0) {
             st->error = error;
             st->iter = st->iter + 1;
             st->done = error < tol ? 1 : 5;
@@ -839,29 +957,28 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, KVec<T> v, const
         return;
     }
     const T beta = (T)(s[1] / st->rr_old);
-    {
-        T* __restrict__ p = v.p;
-        const T* __restrict__ r = v.r;
-        const T* __restrict__ d = v.d;
-        int beg, end;
-        chunk_of(n, beg, end);
-        constexpr int U = 4;
-        int i = beg + threadIdx.x;
-        for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
-            T pi[U], ri[U], di[U];
+    const KVec<T> v = kv_of<T>(S);
+    T* __restrict__ p = v.p;
+    const T* __restrict__ r = v.r;
+    const T* __restrict__ d = v.d;
+    int beg, end;
+    chunk_of(S.n, c.G, c.b, beg, end);
+    constexpr int U = 4;
+    int i = beg + threadIdx.x;
+    for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
+        T pi[U], ri[U], di[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int k = i + u * kBlock;
-                pi[u] = p[k];
-                ri[u] = r[k];
-                di[u] = d[k];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            pi[u] = p[k];
+            ri[u] = r[k];
+            di[u] = d[k];
         }
-        for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+#pragma unroll
+        for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
+    if (c.b == 0 && threadIdx.x == 0) {
         const int it = st->iter + 1;
         st->error = error;
         st->rr = s[1];
@@ -870,47 +987,56 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(int n, KVec<T> v, const
     }
 }
 
-// Reliable update: after the FP64 residual replaced r, partials (r,r) and
-// (r/d, r) of the new r.
+// Reliable update: partials (r,r) and (r/d, r) of the replaced residual...
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_rescan(int n, KVec<T> v, double* part)
+__global__ __launch_bounds__(kBlock) void k_cg_rescan(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const KVec<T> v = kv_of<T>(S);
     double acc[2] = {0.0, 0.0};
-    vec_chunk(n, [&](int i) {
+    vec_chunk(S.n, c.G, c.b, [&](int i) {
         const T ri = v.r[i];
         const T zi = ri / v.d[i];
         acc[0] += (double)ri * (double)ri;
         acc[1] += (double)zi * (double)ri;
     });
-    store_partials<2>(acc, part);
+    store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, c.b);
 }
 
-// ... then the direction step that the update interrupted:
-// beta = (z,r)_new / rr_old, p = z + beta p, and the recurrence resumes.
+// ... then the direction step the update interrupted: beta = (z,r)/rr_old,
+// p = z + beta p, and the recurrence resumes.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_resume(int n, KVec<T> v, const double* part,
-                                                      KrylovState* st)
+__global__ __launch_bounds__(kBlock) void k_cg_resume(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
     double s[2];
-    reduce_partials<2>(part, gridDim.x, s);
+    reduce_partials<2>(S.part + 4 * (size_t)c.G, c.G, s);
+    KrylovState* st = S.st;
     const T beta = (T)(s[1] / st->rr_old);
-    vec_chunk(n, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const KVec<T> v = kv_of<T>(S);
+    vec_chunk(S.n, c.G, c.b, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
+    if (c.b == 0 && threadIdx.x == 0) {
         st->rr = s[1];
         st->done = 0;
     }
 }
 
-// ------------------------------------------------------------------------
-// Right-preconditioned Jacobi BiCGStab (Linear_EQS::BiCGStab :1287)
-// ------------------------------------------------------------------------
+// ---- Right-preconditioned Jacobi BiCGStab (Linear_EQS::BiCGStab :1287)
+// Partial layout: [0,2G) (r,r),(r0,r); [4G,5G) (r0,v); [8G,9G) (s,s);
+// [12G,14G) (t,t),(t,s).
+
 // s = A x; r0 = b - s; r = r0; v = p = 0; partials (r,r), (r0,r), (b,b)
-// (the first two in the layout k_bi_dir reads)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_init(MatView<VT> m, KVec<T> v, double* part)
+__global__ __launch_bounds__(kBlock) void k_bi_init(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const MatView<VT> m = mv_of<VT>(S);
+    const KVec<T> v = kv_of<T>(S);
     double acc[3] = {0.0, 0.0, 0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.x, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.x, c.G, c.b, [&](int i, T ax) {
         const T ri = v.b[i] - ax;
         v.r0[i] = ri;
         v.r[i] = ri;
@@ -920,24 +1046,26 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(MatView<VT> m, KVec<T> v, do
         acc[1] += (double)ri * (double)ri;
         acc[2] += (double)v.b[i] * (double)v.b[i];
     });
-    store_partials<3>(acc, part);
+    store_partials<3>(acc, S.part, c.G, c.b);
 }
 
-__global__ void k_bi_init_final(int G, const double* part, KrylovState* st, double bnorm_prev,
-                                double tol, int max_iter)
+__global__ void k_bi_init_final(Grp g)
 {
+    const SysDev& S = *sys_of_block(g);
     double s[3];
-    reduce_partials<3>(part, G, s);
+    reduce_partials<3>(S.part, S.G, s);
     if (threadIdx.x != 0) return;
+    KrylovState* st = S.st;
+    const KrylovParams P = *S.prm;
     const double bnorm_new = sqrt(s[2]);
     st->iter = 0;
-    st->max_iter = max_iter;
+    st->max_iter = P.max_iter;
     st->done = 0;
     st->rho0 = 1.0;
     st->alpha = 1.0;
     st->omega = 1.0;
-    if (bnorm_prev > 0.0 && bnorm_new / bnorm_prev < tol) {
-        st->error = bnorm_new / bnorm_prev;
+    if (P.bnorm_prev > 0.0 && bnorm_new / P.bnorm_prev < P.tol) {
+        st->error = bnorm_new / P.bnorm_prev;
         st->bnorm = bnorm_new;
         st->done = 4;
         return;
@@ -949,30 +1077,32 @@ __global__ void k_bi_init_final(int G, const double* part, KrylovState* st, doub
         return;
     }
     st->error = sqrt(s[0]) / bnorm_new;
-    if (st->error < tol) st->done = 1;
+    if (st->error < P.tol) st->done = 1;
 }
 
 // B1: test of the previous iterate's r; rho1 = (r0,r); p update; p_h = p/d
-// part_rr: partials [(r,r), (r0,r)] from k_bi_init or k_bi_update.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_dir(int n, KVec<T> v, const double* part_rr,
-                                                   KrylovState* st, double tol)
+__global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
     double s[2];
-    reduce_partials<2>(part_rr, gridDim.x, s);
+    reduce_partials<2>(S.part, c.G, s);
+    const double tol = S.prm->tol;
     const int it = st->iter;
     if (it > 0) {
         const double err = sqrt(s[0]) / st->bnorm;
         if (err < tol || fabs(st->omega) < DBL_MIN) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (c.b == 0 && threadIdx.x == 0) {
                 st->error = err;
                 st->done = (err < tol) ? 1 : 3;
             }
             return;
         }
         if (it >= st->max_iter) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (c.b == 0 && threadIdx.x == 0) {
                 st->error = err;
                 st->done = 2;
             }
@@ -981,11 +1111,12 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(int n, KVec<T> v, const doubl
     }
     const double rho1 = s[1];
     if (fabs(rho1) < DBL_MIN) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
+        if (c.b == 0 && threadIdx.x == 0) st->done = 6;
         return;
     }
+    const KVec<T> v = kv_of<T>(S);
     if (it == 0) {
-        vec_chunk(n, [&](int i) {
+        vec_chunk(S.n, c.G, c.b, [&](int i) {
             const T pi = v.r[i];
             v.p[i] = pi;
             v.ph[i] = pi / v.d[i];
@@ -993,50 +1124,57 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(int n, KVec<T> v, const doubl
     } else {
         const T beta = (T)((rho1 / st->rho0) * (st->alpha / st->omega));
         const T omega = (T)st->omega;
-        vec_chunk(n, [&](int i) {
+        vec_chunk(S.n, c.G, c.b, [&](int i) {
             const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
             v.p[i] = pi;
             v.ph[i] = pi / v.d[i];
         });
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->rho1 = rho1;
+    if (c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
 }
 
 // B2: v = A p_h; partial (r0,v); counts the iteration
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(MatView<VT> m, KVec<T> v, double* part,
-                                                      KrylovState* st)
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
+    const MatView<VT> m = mv_of<VT>(S);
+    const KVec<T> v = kv_of<T>(S);
     double acc[1] = {0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.ph, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.ph, c.G, c.b, [&](int i, T ax) {
         v.v[i] = ax;
         acc[0] += (double)v.r0[i] * (double)ax;
     });
-    store_partials<1>(acc, part);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+    store_partials<1>(acc, S.part + 4 * (size_t)c.G, c.G, c.b);
+    if (c.b == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
 // B3: alpha = rho1/(r0,v); s = r - alpha v; s_h = s/d; partial (s,s)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_s(int n, KVec<T> v, const double* part_r0v,
-                                                 double* part, KrylovState* st)
+__global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
-    double s[1];
-    reduce_partials<1>(part_r0v, gridDim.x, s);
+    double sv[1];
+    reduce_partials<1>(S.part + 4 * (size_t)c.G, c.G, sv);
     const double rho1 = st->rho1;
-    const double alpha_d = rho1 / s[0];
+    const double alpha_d = rho1 / sv[0];
     const T alpha = (T)alpha_d;
+    const KVec<T> v = kv_of<T>(S);
     double acc[1] = {0.0};
-    vec_chunk(n, [&](int i) {
+    vec_chunk(S.n, c.G, c.b, [&](int i) {
         const T si = v.r[i] - alpha * v.v[i];
         v.s[i] = si;
         v.sh[i] = si / v.d[i];
         acc[0] += (double)si * (double)si;
     });
-    store_partials<1>(acc, part);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    store_partials<1>(acc, S.part + 8 * (size_t)c.G, c.G, c.b);
+    if (c.b == 0 && threadIdx.x == 0) {
         st->alpha = alpha_d;
         st->rho0 = rho1;
     }
@@ -1045,98 +1183,103 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, KVec<T> v, const double*
 // B4: test on s (then x += alpha p_h and stop); else t = A s_h;
 // partials (t,t), (t,s)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(MatView<VT> m, KVec<T> v,
-                                                      const double* part_ss, double* part,
-                                                      KrylovState* st, double tol)
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(Grp g)
 {
-    const int n = m.n;
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
     double ss[1];
-    reduce_partials<1>(part_ss, gridDim.x, ss);
+    reduce_partials<1>(S.part + 8 * (size_t)c.G, c.G, ss);
     const double err = sqrt(ss[0]) / st->bnorm;
-    if (err < tol) {
+    const KVec<T> v = kv_of<T>(S);
+    if (err < S.prm->tol) {
         const T alpha = (T)st->alpha;
-        vec_chunk(n, [&](int i) { v.x[i] += alpha * v.ph[i]; });
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        vec_chunk(S.n, c.G, c.b, [&](int i) { v.x[i] += alpha * v.ph[i]; });
+        if (c.b == 0 && threadIdx.x == 0) {
             st->error = err;
             st->done = 1;
         }
         return;
     }
+    const MatView<VT> m = mv_of<VT>(S);
     double acc[2] = {0.0, 0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.sh, [&](int i, T ax) {
+    spmv_rows<FMT, VT, T, T>(m, v.sh, c.G, c.b, [&](int i, T ax) {
         v.t[i] = ax;
         acc[0] += (double)ax * (double)ax;
         acc[1] += (double)ax * (double)v.s[i];
     });
-    store_partials<2>(acc, part);
+    store_partials<2>(acc, S.part + 12 * (size_t)c.G, c.G, c.b);
 }
 
 // B5: omega = (t,s)/(t,t); x += alpha p_h + omega s_h; r = s - omega t;
 // partials (r,r), (r0,r)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_update(int n, KVec<T> v, const double* part_t,
-                                                      double* part, KrylovState* st)
+__global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    KrylovState* st = S.st;
     if (st->done) return;
-    double s[2];
-    reduce_partials<2>(part_t, gridDim.x, s);
-    const double omega_d = (s[0] > DBL_MIN) ? s[1] / s[0] : 1.0;
+    double sv[2];
+    reduce_partials<2>(S.part + 12 * (size_t)c.G, c.G, sv);
+    const double omega_d = (sv[0] > DBL_MIN) ? sv[1] / sv[0] : 1.0;
     const T omega = (T)omega_d;
     const T alpha = (T)st->alpha;
+    const KVec<T> v = kv_of<T>(S);
     double acc[2] = {0.0, 0.0};
-    vec_chunk(n, [&](int i) {
+    vec_chunk(S.n, c.G, c.b, [&](int i) {
         v.x[i] += alpha * v.ph[i] + omega * v.sh[i];
         const T ri = v.s[i] - omega * v.t[i];
         v.r[i] = ri;
         acc[0] += (double)ri * (double)ri;
         acc[1] += (double)v.r0[i] * (double)ri;
     });
-    store_partials<2>(acc, part);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->omega = omega_d;
+    store_partials<2>(acc, S.part, c.G, c.b);
+    if (c.b == 0 && threadIdx.x == 0) st->omega = omega_d;
 }
 
-// ------------------------------------------------------------------------
-// FP32-inside-FP64 iterative refinement helpers
-// ------------------------------------------------------------------------
-// r = b - A x in FP64; partials (b,b), (r,r)
+// ---- FP32-inside-FP64 iterative refinement helpers
+
+// r64 = b - A x in FP64; partials (b,b), (r,r)
 template <int FMT>
-__global__ __launch_bounds__(kBlock) void k_ir_residual(MatView<double> m, const double* x,
-                                                        const double* b, double* r, double* part)
+__global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
 {
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const MatView<double> m = S.m64;
+    const KVec<double> v = S.v64;
+    double* r = S.r64;
     double acc[2] = {0.0, 0.0};
-    spmv_rows<FMT, double, double, double>(m, x, [&](int i, double ax) {
-        const double ri = b[i] - ax;
+    spmv_rows<FMT, double, double, double>(m, v.x, c.G, c.b, [&](int i, double ax) {
+        const double ri = v.b[i] - ax;
         r[i] = ri;
-        acc[0] += b[i] * b[i];
+        acc[0] += v.b[i] * v.b[i];
         acc[1] += ri * ri;
     });
-    store_partials<2>(acc, part);
+    store_partials<2>(acc, S.part, c.G, c.b);
 }
 
-__global__ void k_reduce_to(int G, const double* part, int nv, double* out)
+// r32 (into b32 when into_b, else r32) = scale * r64; x32 = 0
+__global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g, int into_b)
 {
-    double s[2];
-    reduce_partials<2>(part, G, s);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < nv && k < 2; k++) out[k] = s[k];
-}
-
-// r32 = r64 * scale; x32 = 0
-__global__ __launch_bounds__(kBlock) void k_ir_prepare(int n, const double* r, double scale,
-                                                       float* r32, float* x32)
-{
-    vec_chunk(n, [&](int i) {
-        r32[i] = (float)(r[i] * scale);
-        x32[i] = 0.0f;
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const double scale = S.prm->scale;
+    float* dst = into_b ? S.v32.b : S.v32.r;
+    vec_chunk(S.n, c.G, c.b, [&](int i) {
+        dst[i] = (float)(S.r64[i] * scale);
+        S.v32.x[i] = 0.0f;
     });
 }
 
-// x += scale * d32
-__global__ __launch_bounds__(kBlock) void k_ir_correct(int n, double* x, const float* d32,
-                                                       double scale)
+// x64 += unscale * x32
+__global__ __launch_bounds__(kBlock) void k_ir_correct(Grp g)
 {
-    vec_chunk(n, [&](int i) { x[i] += scale * (double)d32[i]; });
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const double u = S.prm->unscale;
+    vec_chunk(S.n, c.G, c.b, [&](int i) { S.v64.x[i] += u * (double)S.v32.x[i]; });
 }
 
 // ------------------------------------------------------------------------
@@ -1159,32 +1302,12 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(int64_t n4, const double
         b[i] = a[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_cast_f32(int64_t n, const double* a, float* b)
-{
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock)
-        b[i] = (float)a[i];
-}
-
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const T* a, T* b)
 {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock)
         b[i] = a[i];
-}
-
-// Jacobi diagonal: Precond_Jacobi's a_ii with |a_ii| < DBL_MIN -> 1
-// (FEM/matrix_class.cpp:1786-1821); no preconditioner -> 1.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_diag(int n, const int* diagpos, const double* A,
-                                                 T* d, int jacobi)
-{
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        double v = (jacobi && diagpos[i] >= 0) ? A[diagpos[i]] : 1.0;
-        if (fabs(v) < DBL_MIN) v = 1.0;
-        d[i] = (T)v;
-    }
 }
 
 // Point source terms: one thread per distinct node, values in list order.

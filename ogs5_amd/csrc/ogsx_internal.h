@@ -87,8 +87,9 @@ struct KrylovState {
     double rho0;      // BiCGStab: rho of the previous iteration
     double rho1;      // BiCGStab: rho of this iteration
     int iter;         // completed iterations
-    int done;         // 0 running, 1 converged, 2 max_iter, 3 breakdown,
-                      // 4 skipped by CheckNormRHS
+    int done;         // 0 running, 1 converged, 2 max_iter, 3 breakdown
+                      // (BiCGStab: omega), 4 skipped by CheckNormRHS,
+                      // 5 reliable-update request, 6 BiCGStab rho breakdown
     int max_iter;
     int pad;
 };

@@ -38,18 +38,17 @@ def gaussian_field_grid(shape, h, corr_len: float, sigma: float, seed: int,
     rng = np.random.Generator(np.random.PCG64(seed))
     k, phi = _wave_vectors(rng, dim, n_modes, corr_len)
     centres = [(np.arange(shape[a]) + 0.5) * h[a] for a in range(dim)]
-    out = np.zeros(tuple(reversed(shape)), dtype=np.float64)  # [..., z, y, x]
-    for m in range(n_modes):
-        term = np.exp(1j * phi[m])
-        acc = None
-        for a in range(dim):
-            e = np.exp(1j * k[m, a] * centres[a])
-            # axis a is the (dim-1-a)-th axis of `out`
-            sh = [1] * dim
-            sh[dim - 1 - a] = shape[a]
-            e = e.reshape(sh)
-            acc = e if acc is None else acc * e
-        out += (acc * term).real
+    amp = np.exp(1j * phi)  # [M]
+    # E[a][m, i] = exp(i k_ma x_i): the mode sum factorises per axis
+    E = [np.exp(1j * np.outer(k[:, a], centres[a])) for a in range(dim)]
+    if dim == 1:
+        out = (amp @ E[0]).real
+    elif dim == 2:
+        out = ((E[1].T * amp) @ E[0]).real  # [y, x]
+    else:
+        out = np.empty((shape[2], shape[1], shape[0]))
+        for z in range(shape[2]):
+            out[z] = ((E[1].T * (amp * E[2][:, z])) @ E[0]).real
     return (sigma * np.sqrt(2.0 / n_modes) * out).ravel()
 
 

@@ -147,6 +147,20 @@ def tet_heat(n: int = 96, *, seed: int = 3, conductivity: float = 2.0, rho_c: fl
                     meta=dict(seed=seed, peclet_min=float(pe.min()), peclet_max=float(pe.max())))
 
 
+def ensemble(count: int = 256, *, side_range=(64, 512), sigma_range=(0.5, 2.5),
+             n_steps: int = 10, seed: int = 0) -> List[Scenario]:
+    """Independent 2-D flow instances with the quad_flow set-up (1000 m
+    square, Dirichlet heads on the x edges, centre sink, 10 implicit steps):
+    side s = round(exp(U[ln lo, ln hi])) cells, sigma_lnK ~ U[sigma_range],
+    instance i seeded with i."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = side_range
+    sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    sigmas = rng.uniform(sigma_range[0], sigma_range[1], count)
+    return [quad_flow(int(s), int(s), seed=i, sigma_ln_k=float(sg), n_steps=n_steps)
+            for i, (s, sg) in enumerate(zip(sides, sigmas))]
+
+
 def bmskel() -> Scenario:
     """The reference's tests/data/bmskel deck: one line element of length 1,
     HEAT_TRANSPORT, T0 = 273, Dirichlet 274 at POINT0 (node 0), one step of
@@ -180,15 +194,8 @@ def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SY
     s.set_vector(L.VEC_UOLD, sc.u0)
     s.set_vector(L.VEC_X, sc.u0)
     infos, xs = [], []
-    for st in sc.steps:
-        if st.reset_to_ic:
-            s.set_vector(L.VEC_UOLD, sc.u0)
-            s.set_vector(L.VEC_X, sc.u0)
-        s.initialize()
-        s.assemble(st.dt, st.theta, st.steady)
-        if sc.st_idx.size:
-            s.add_rhs(sc.st_idx, sc.st_val)
-        s.set_dirichlet(sc.bc_idx, sc.bc_val, dirichlet)
+    for k, st in enumerate(sc.steps):
+        _prepare_step(sc, s, k, dirichlet)
         info = s.solve(method=method, tol=sc.tol if tol is None else tol,
                        max_iter=sc.max_iter, inner_tol=inner_tol, ir_mode=ir_mode)
         infos.append(info)
@@ -197,6 +204,63 @@ def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SY
             xs.append(s.get_vector(L.VEC_X))
     x = s.get_vector(L.VEC_X)
     out = dict(x=x, info=infos, system=(pat, s))
+    if keep_steps:
+        out["steps"] = xs
+    return out
+
+
+def _prepare_step(sc: Scenario, s, k: int, dirichlet: int):
+    st = sc.steps[k]
+    if st.reset_to_ic:
+        s.set_vector(L.VEC_UOLD, sc.u0)
+        s.set_vector(L.VEC_X, sc.u0)
+    s.initialize()
+    s.assemble(st.dt, st.theta, st.steady)
+    if sc.st_idx.size:
+        s.add_rhs(sc.st_idx, sc.st_val)
+    s.set_dirichlet(sc.bc_idx, sc.bc_val, dirichlet)
+
+
+def run_group(scs: List[Scenario], *, method: int = L.CG,
+              dirichlet: int = L.DIRICHLET_SYMMETRIC, precision: int = L.FP64,
+              inner_tol: Optional[float] = None, tol: Optional[float] = None,
+              ir_mode: Optional[int] = None, use_graphs: bool = True, keep_steps=False,
+              stagger: int = 0, systems=None) -> dict:
+    """Runs the scenarios' steps in lock-step through one ogsx group: per
+    step every instance is assembled on its own stream, then all solves run
+    together.  stagger > 0 starts the instances one at a time with that many
+    group iterations in between (solves join while others are running)."""
+    if systems is None:
+        systems = [make_system(sc, precision) for sc in scs]
+    g = L.Group(precision, use_graphs)
+    for sc, (_, s) in zip(scs, systems):
+        g.add(s)
+        s.set_vector(L.VEC_UOLD, sc.u0)
+        s.set_vector(L.VEC_X, sc.u0)
+    infos = [[] for _ in scs]
+    xs = [[] for _ in scs]
+    for k in range(max(len(sc.steps) for sc in scs)):
+        live = [i for i, sc in enumerate(scs) if k < len(sc.steps)]
+        for i in live:
+            _prepare_step(scs[i], systems[i][1], k, dirichlet)
+        for i in live:
+            sc = scs[i]
+            g.start([systems[i][1]], method=method, tol=sc.tol if tol is None else tol,
+                    max_iter=sc.max_iter, inner_tol=inner_tol, ir_mode=ir_mode)
+            if stagger:
+                g.advance(stagger)
+        while g.advance() > 0:
+            pass
+        for i in live:
+            s = systems[i][1]
+            active, info = L.solve_status(s)
+            assert not active
+            infos[i].append(info)
+            s.copy_vector(L.VEC_UOLD, L.VEC_X)
+            if keep_steps:
+                xs[i].append(s.get_vector(L.VEC_X))
+    out = dict(x=[s.get_vector(L.VEC_X) for _, s in systems], info=infos, systems=systems,
+               group=g)
     if keep_steps:
         out["steps"] = xs
     return out

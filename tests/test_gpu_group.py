@@ -1,0 +1,94 @@
+"""Systems solved together through an ogsx group (batched ensemble and
+continuous batching): every instance's iterates, iteration counts and
+solutions must be bit-identical to the same instance solved alone, and the
+solutions must meet the oracle's bar (1e-10 relative L2)."""
+import numpy as np
+import pytest
+
+import ogs5_amd as ox
+from ogs5_amd import _lib as L
+from ogs5_amd import workloads as W
+from oracle import driver
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    """Uneven 2-D and 3-D flow instances (different sizes, seeds, sigma)."""
+    scs = [W.quad_flow(n, n, seed=20 + n, sigma_ln_k=s, n_steps=3)
+           for n, s in [(9, 0.5), (24, 1.0), (40, 2.0), (63, 1.5), (100, 2.5)]]
+    scs += [W.hex_flow(n, seed=30 + n, n_transient=2, sigma_ln_k=1.5) for n in (5, 12)]
+    return scs
+
+
+MIX = _mix()
+
+
+def _same(a, b):
+    assert len(a["info"]) == len(b["info"])
+    for ia, ib in zip(a["info"], b["info"]):
+        assert ia["iterations"] == ib["iterations"], (ia, ib)
+        assert ia["converged"] == ib["converged"]
+        assert ia["error"] == ib["error"]
+        assert ia["ret"] == ib["ret"]
+    for xa, xb in zip(a["steps"], b["steps"]):
+        assert np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("prec", [ox.FP64, ox.FP32_IR])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_group_bit_identical_to_alone(prec, graphs):
+    alone = [W.run_gpu(sc, precision=prec, keep_steps=True) for sc in MIX]
+    mix = W.run_group(MIX, precision=prec, use_graphs=graphs, keep_steps=True)
+    print(f"[group] prec={prec} graphs={graphs} iters="
+          f"{[[i['iterations'] for i in inf] for inf in mix['info']]}", flush=True)
+    for k, a in enumerate(alone):
+        _same(a, dict(info=mix["info"][k], steps=mix["steps"][k]))
+        assert all(i["converged"] == 1 for i in mix["info"][k])
+
+
+@pytest.mark.parametrize("prec", [ox.FP64, ox.FP32_IR])
+def test_continuous_batching_bit_identical(prec):
+    """Instances join one at a time while earlier ones are mid-solve."""
+    alone = [W.run_gpu(sc, precision=prec, keep_steps=True) for sc in MIX]
+    mix = W.run_group(MIX, precision=prec, keep_steps=True, stagger=7)
+    for k, a in enumerate(alone):
+        _same(a, dict(info=mix["info"][k], steps=mix["steps"][k]))
+
+
+def test_group_bicgstab_heat_bit_identical():
+    scs = [W.tet_heat(n, seed=40 + n, n_steps=2) for n in (3, 5, 7)]
+    for prec in (ox.FP64, ox.FP32_IR):
+        alone = [W.run_gpu(sc, method=ox.BICGSTAB, dirichlet=ox.DIRICHLET_ROW, precision=prec,
+                           keep_steps=True) for sc in scs]
+        mix = W.run_group(scs, method=ox.BICGSTAB, dirichlet=ox.DIRICHLET_ROW, precision=prec,
+                          keep_steps=True, stagger=3)
+        for k, a in enumerate(alone):
+            _same(a, dict(info=mix["info"][k], steps=mix["steps"][k]))
+
+
+def test_group_solutions_vs_oracle():
+    """Group FP32-IR solutions of the mix against a tight FP64 oracle CG."""
+    from oracle import ffi
+    mix = W.run_group(MIX[:4], precision=ox.FP32_IR, tol=1e-13, keep_steps=True)
+    for k, sc in enumerate(MIX[:4]):
+        ref = driver.run_oracle(sc, ffi, method=5, symmetric=True, tol=1e-14, keep_steps=True)
+        for x, xr in zip(mix["steps"][k], ref["steps"]):
+            assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-10
+
+
+def test_group_membership_rules():
+    sc = MIX[0]
+    pat, s = W.make_system(sc)
+    g = L.Group(ox.FP64)
+    g.add(s)
+    with pytest.raises(ox.OgsxError):
+        s.solve()  # members solve through their group
+    with pytest.raises(ox.OgsxError):
+        g.add(s)  # one group per system
+    pat2, s2 = W.make_system(sc, ox.FP32_IR)
+    with pytest.raises(ox.OgsxError):
+        g.add(s2)  # precision differs
+    g.remove(s)
+    W.run_gpu(sc, system=(pat, s))  # alone again
+    g.close()
