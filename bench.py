@@ -137,6 +137,115 @@ def spmv_mode(size):
                       "variant64": s.spmv_variant(), "tuning": s.spmv_tuning()}))
 
 
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ensemble_mode(count, steps):
+    """Batched ensemble: `count` independent 2-D flow instances (quad side
+    log-uniform 64-512, sigma_lnK 0.5-2.5), `steps` implicit steps each.
+    Three timed runs of the whole job: one group in FP32-IR and in FP64
+    (one launch per Krylov phase over all running instances, iteration
+    batches replayed as HIP graphs), and the plain method (ogsx_solve on one
+    instance at a time, FP64).  Untimed warm-up: every instance's first
+    step (SpMV format tuning, graph capture)."""
+    import ogs5_amd as ox
+    from ogs5_amd import workloads as W
+    ox.device_count()
+    scs = W.ensemble(count, n_steps=steps)
+    total = sum(sc.n for sc in scs) * steps
+    warm = [W.Scenario(**{**sc.__dict__, "steps": sc.steps[:1]}) for sc in scs]
+    res = {}
+    for name, prec in (("group_fp32_ir", ox.FP32_IR), ("group_fp64", ox.FP64)):
+        systems = [W.make_system(sc, prec) for sc in scs]
+        W.run_group(warm, precision=prec, systems=systems)
+        for _, s in systems:
+            s.synchronize()
+        t0 = time.perf_counter()
+        out = W.run_group(scs, precision=prec, systems=systems)
+        for _, s in systems:
+            s.synchronize()
+        dt = time.perf_counter() - t0
+        its = [i["iterations"] for inf in out["info"] for i in inf]
+        assert all(i["converged"] == 1 and i["error"] <= 1e-10 for inf in out["info"] for i in inf)
+        res[name] = dict(seconds=dt, mdof_solves_per_s=total / dt / 1e6,
+                         iterations_max=max(its), iterations_mean=float(np.mean(its)))
+        _log(f"ensemble {name}: {dt:.2f} s, {total / dt / 1e6:.1f} MDOF-solves/s")
+        out["group"].close()
+        if prec == ox.FP64:
+            for sc, sysw in zip(warm, systems):
+                W.run_gpu(sc, precision=ox.FP64, system=sysw)
+            t0 = time.perf_counter()
+            its = []
+            for sc, sysw in zip(scs, systems):
+                r = W.run_gpu(sc, precision=ox.FP64, system=sysw)
+                its += [i["iterations"] for i in r["info"]]
+            dt = time.perf_counter() - t0
+            res["plain_fp64"] = dict(seconds=dt, mdof_solves_per_s=total / dt / 1e6,
+                                     iterations_max=max(its), iterations_mean=float(np.mean(its)))
+            _log(f"ensemble plain_fp64: {dt:.2f} s, {total / dt / 1e6:.1f} MDOF-solves/s")
+        del systems
+    best = res["group_fp32_ir"]
+    print(json.dumps({
+        "metric": "MDOF-solves/s to 1e-10 FP64 resid",
+        "value": best["mdof_solves_per_s"], "unit": "MDOF-solves/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": 1e3 * best["seconds"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-in-f64-IR", "data": "synthetic (seeded lognormal K per instance)",
+        "config": {"workload": f"batched ensemble: {count} 2-D flow instances, quad side "
+                               f"log-uniform 64-512, {steps} implicit steps each, "
+                               "Jacobi-PCG to 1e-10",
+                   "dof_total": sum(sc.n for sc in scs)},
+        "runs": res,
+        "speedup_vs_plain_fp64": best["mdof_solves_per_s"] / res["plain_fp64"]["mdof_solves_per_s"],
+        "speedup_vs_group_fp64": best["mdof_solves_per_s"] / res["group_fp64"]["mdof_solves_per_s"],
+    }))
+
+
+def serve_runs(count, rate, plain=True):
+    """Served stream: `count` requests, Poisson arrivals at `rate`/s, 2-D
+    quads 32^2-512^2 and 3-D hexes 16^3-96^3, 1-20 steps each.  The
+    continuous-batching server in FP32-IR (and FP64), and the plain method
+    (one request at a time, solo FP64 solves) on the same stream."""
+    from ogs5_amd import _lib as L
+    from ogs5_amd import workloads as W
+    from ogs5_amd.serve import Server, serve_sequential
+    specs = W.request_stream(count, rate)
+    res = {}
+    runs = [("batched_fp32_ir", L.FP32_IR), ("batched_fp64", L.FP64)]
+    for name, prec in runs:
+        out = Server(prec).serve(specs)
+        out.pop("jobs")
+        res[name] = out
+        _log(f"serve {name}: {out['requests_per_s']:.1f} req/s, "
+             f"{out['mdof_solves_per_s']:.1f} MDOF-solves/s, p50 {out['latency_ms']['p50']:.0f} ms, "
+             f"p99 {out['latency_ms']['p99']:.0f} ms")
+    if plain:
+        out = serve_sequential(specs)
+        out.pop("jobs")
+        res["plain_fp64"] = out
+        _log(f"serve plain_fp64: {out['requests_per_s']:.1f} req/s, p50 "
+             f"{out['latency_ms']['p50']:.0f} ms, p99 {out['latency_ms']['p99']:.0f} ms")
+    return res
+
+
+def serve_mode(count, rate):
+    res = serve_runs(count, rate)
+    best = res["batched_fp32_ir"]
+    print(json.dumps({
+        "metric": "served p50/p99 latency; MDOF-solves/s to 1e-10 FP64 resid",
+        "value": best["mdof_solves_per_s"], "unit": "MDOF-solves/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": 1e3 * best["makespan_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-in-f64-IR", "data": "synthetic request stream (seeded)",
+        "config": {"workload": f"served stream: {count} requests, Poisson {rate}/s, quads "
+                               "32^2-512^2 and hexes 16^3-96^3, 1-20 steps, Jacobi-PCG to 1e-10"},
+        "served_p50_ms": best["latency_ms"]["p50"], "served_p99_ms": best["latency_ms"]["p99"],
+        "runs": res,
+        "speedup_vs_plain_fp64": best["mdof_solves_per_s"] / res["plain_fp64"]["mdof_solves_per_s"],
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +253,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["full", "spmv"], default="full")
+    ap.add_argument("--mode", choices=["full", "spmv", "ensemble", "serve"], default="full")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=10)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--serve-requests", type=int, default=100,
+                    help="requests of the short served run in the full mode (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -152,6 +267,12 @@ def main():
         sys.exit(2)
     if args.mode == "spmv":
         spmv_mode(args.size)
+        return
+    if args.mode == "ensemble":
+        ensemble_mode(args.instances, args.ens_steps)
+        return
+    if args.mode == "serve":
+        serve_mode(args.requests, args.rate)
         return
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
@@ -202,6 +323,15 @@ def main():
         "served_p50_ms": None,
         "served_p99_ms": None,
     }
+    if args.serve_requests > 0:
+        sv = serve_runs(args.serve_requests, args.rate, plain=False)["batched_fp32_ir"]
+        out["served_p50_ms"] = sv["latency_ms"]["p50"]
+        out["served_p99_ms"] = sv["latency_ms"]["p99"]
+        out["served"] = dict(requests=sv["requests"], rate_per_s=args.rate,
+                             requests_per_s=sv["requests_per_s"],
+                             mdof_solves_per_s=sv["mdof_solves_per_s"],
+                             note="short served run (continuous batching, FP32-IR); "
+                                  "bench.py --mode serve runs the full comparison")
     print(json.dumps(out))
 
 

@@ -102,6 +102,11 @@ int ogsx_pattern_export(const ogsx_pattern* p, int64_t* row_ptr, int64_t* col,
                         int64_t* diag);
 /* Element colour of every element (n_elem ints). */
 int ogsx_pattern_colors(const ogsx_pattern* p, int32_t* color);
+/* Device bytes that ogsx_system_create + ogsx_system_set_elements (without
+ * Gauss-point velocities) allocate for a system on this pattern: what a
+ * caller that admits work by HBM footprint reserves before creating it.
+ * No counterpart in the reference (its matrices live in host memory). */
+int ogsx_pattern_footprint(const ogsx_pattern* p, int precision, int64_t* bytes);
 int ogsx_pattern_destroy(ogsx_pattern* p);
 
 /* ---- system: Linear_EQS + the element data it is assembled from -------- */
@@ -109,6 +114,8 @@ int ogsx_pattern_destroy(ogsx_pattern* p);
  * on the pattern), b, x and Krylov buffers; bNorm starts at 1.0. */
 int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out);
 int ogsx_system_destroy(ogsx_system* s);
+/* Device bytes the system has allocated so far. */
+int ogsx_system_bytes(const ogsx_system* s, int64_t* bytes);
 
 /* Element data for CFiniteElementStd::Assembly (FEM/fem_ele_std.cpp:9367).
  * Uploaded once; later assemblies reuse it. */

@@ -6,11 +6,12 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libogsx.so")
+LIB_PATH = os.environ.get("OGSX_LIB") or os.path.join(_HERE, "libogsx.so")
 
 OK = 0
 LINE2, QUAD4, HEX8, TET4 = 1, 2, 3, 5
@@ -90,6 +91,8 @@ SIGNATURES = {
                                   C.POINTER(C.c_double)]),
     "ogsx_stream_copy": (C.c_int, [_I64, C.c_int, C.POINTER(C.c_double)]),
     "ogsx_synchronize": (C.c_int, [_P]),
+    "ogsx_pattern_footprint": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
+    "ogsx_system_bytes": (C.c_int, [_P, C.POINTER(_I64)]),
     "ogsx_group_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P)]),
     "ogsx_group_destroy": (C.c_int, [_P]),
     "ogsx_group_add": (C.c_int, [_P, _P]),
@@ -101,12 +104,17 @@ SIGNATURES = {
 }
 
 _lib = None
+_lib_lock = threading.Lock()
 
 
 def lib():
     """The loaded libogsx.so (raises if it was not built)."""
     global _lib
-    if _lib is None:
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
         if not os.path.exists(LIB_PATH):
             raise OgsxError(f"{LIB_PATH} is missing: run __graft_entry__.build() first "
                             "(the HIP library has no CPU fallback)")
@@ -189,6 +197,13 @@ class Pattern:
         c = np.empty(self.n_elem, np.int32)
         check(lib().ogsx_pattern_colors(self.h, _ptr(c)), "colors")
         return c
+
+    def footprint(self, precision: int = FP64) -> int:
+        """Device bytes of a system on this pattern (ogsx_pattern_footprint)."""
+        b = C.c_int64()
+        check(lib().ogsx_pattern_footprint(self.h, int(precision), C.byref(b)),
+              "ogsx_pattern_footprint")
+        return b.value
 
     def close(self):
         if self.h:
@@ -312,6 +327,11 @@ class System:
 
     def synchronize(self):
         check(lib().ogsx_synchronize(self.h), "ogsx_synchronize")
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        check(lib().ogsx_system_bytes(self.h, C.byref(b)), "ogsx_system_bytes")
+        return b.value
 
     def close(self):
         if getattr(self, "h", None):

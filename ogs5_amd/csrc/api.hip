@@ -34,9 +34,24 @@ int fail(int code, const std::string& msg)
             return fail(OGSX_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Blocks of the n-sized kernels of one system (each block a contiguous chunk
+// of rows).  G sets the partial-sum layout, so it is a function of n alone,
+// never tuned at run time: one row per thread up to OGSX_FILL_BLOCKS blocks,
+// then OGSX_ROWS_PER_BLOCK rows per block, at most 2048 blocks.
+// Few rows per block keep mid-size systems (latency-bound) spread over the
+// CUs; many rows per block cut block dispatches and the per-block reduction
+// of partial sums for large systems and for groups of many systems.
+#ifndef OGSX_ROWS_PER_BLOCK
+#define OGSX_ROWS_PER_BLOCK 1024
+#endif
+#ifndef OGSX_FILL_BLOCKS
+#define OGSX_FILL_BLOCKS 256
+#endif
+
 int grid_for(int64_t n)
 {
-    int64_t g = (n + 255) / 256;
+    int64_t g = (n + OGSX_ROWS_PER_BLOCK - 1) / OGSX_ROWS_PER_BLOCK;
+    g = std::max<int64_t>(g, std::min<int64_t>((n + kBlock - 1) / kBlock, OGSX_FILL_BLOCKS));
     g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
     return (int)std::max<int64_t>(kNumXcd, std::min<int64_t>(g, 2048));
 }
@@ -48,8 +63,23 @@ int blocks_for(int64_t n, int per)
 }
 }  // namespace
 
+// Device-ready arrays of a pattern (32-bit CRS, row tiles, SELL-64 layout),
+// built on the host together with the pattern, so that creating a system is
+// allocation and upload only.
+struct PatternLayout {
+    std::vector<int> ptr32, col32, dg32;
+    std::vector<int> trow, tcb;
+    std::vector<uint16_t> tc16;
+    bool tile16 = false;
+    std::vector<int> sptr, sw, scb, sc32, ssrc;
+    std::vector<uint16_t> sc16;
+    bool sell16 = false;
+    int64_t npad = 0;
+};
+
 struct ogsx_pattern {
     PatternHost h;
+    PatternLayout L;
 };
 
 template <typename T>
@@ -67,24 +97,15 @@ struct ProfAcc {
 namespace {
 struct MemberSet;
 
-enum : int { kPhaseKrylov = 0, kPhaseResidual = 1 };
-
-// Host-side progress of one system's solve (one member of a solve set).
+// Host-side record of one system's solve (one member of a solve set).
 struct SolveCtx {
-    int active = 0;           // solve in progress
+    int active = 0;  // solve in progress
     int method = OGSX_CG;
     ogsx_solver_config cfg{};
-    bool ir = false;          // FP32 recurrence inside FP64 refinement
-    bool reliable = false;    // reliable-update CG (else restarted IR)
-    double delta = 0.0;       // inner tolerance / reliable-update threshold
-    int max_outer = 50;
-    int phase = kPhaseKrylov; // IR: waiting for the FP64 residual step
-    int inner_started = 0;    // IR: an FP32 recurrence has run
-    int inner_status = 0;     // IR: how the last recurrence stopped (1 conv/update, 2 max_iter, 3 breakdown)
-    int last_inner_iters = 0;
+    bool ir = false;  // FP32 recurrence inside FP64 refinement
     int total = 0, updates = 0;
     int converged = 0, ret = 0;
-    double error = 0.0, bnorm = 0.0, maxr = 1.0;
+    double error = 0.0, bnorm = 0.0;
 };
 }  // namespace
 
@@ -131,6 +152,8 @@ struct ogsx_system {
     SysDev* dev = nullptr;
     KrylovParams* prm = nullptr;
     KrylovParams* prm_host = nullptr;  // pinned
+    IRState* irs = nullptr;            // IR systems
+    IRState* irs_host = nullptr;       // pinned
     SolveCtx sc;
     MemberSet* solo = nullptr;         // set of one for ogsx_solve
     ogsx_group* group = nullptr;       // group the system belongs to, if any
@@ -161,6 +184,7 @@ struct ogsx_system {
     size_t ev_next = 0;
     ProfAcc prof_acc[3];
     std::vector<void*> allocs;
+    int64_t bytes = 0;  // device bytes allocated
 };
 
 namespace {
@@ -174,6 +198,7 @@ int dalloc(ogsx_system* s, T** p, size_t count)
     if (e != hipSuccess)
         return fail(OGSX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
     s->allocs.push_back(q);
+    s->bytes += (int64_t)(count * sizeof(T));
     *p = (T*)q;
     return OGSX_OK;
 }
@@ -315,6 +340,46 @@ void build_tiles(const PatternHost& h, std::vector<int>& trow, std::vector<int>&
         for (size_t t = 0; t + 1 < trow.size(); t++)
             for (int64_t k = h.ptr[trow[t]]; k < h.ptr[trow[t + 1]]; k++)
                 c16[k] = (uint16_t)(h.col[k] - tcb[t]);
+}
+
+void build_layout(const PatternHost& h, PatternLayout& L)
+{
+    L.ptr32.resize(h.n + 1);
+    L.col32.resize(h.nnz);
+    L.dg32.resize(h.n);
+    for (int64_t i = 0; i <= h.n; i++) L.ptr32[i] = (int)h.ptr[i];
+    for (int64_t k = 0; k < h.nnz; k++) L.col32[k] = (int)h.col[k];
+    for (int64_t i = 0; i < h.n; i++) L.dg32[i] = (int)h.diag[i];
+    build_tiles(h, L.trow, L.tcb, L.tc16, L.tile16);
+    if (!L.tile16) L.tc16.clear();
+    build_sell(h, L.sptr, L.sw, L.scb, L.sc32, L.sc16, L.ssrc, L.sell16);
+    L.npad = L.sptr.back();
+}
+
+// Device bytes a system on pattern p allocates: ogsx_system_create plus
+// ogsx_system_set_elements without Gauss-point velocities (counted as the
+// allocations ask for them, one element for an empty array).
+int64_t system_bytes(const ogsx_pattern* p, int precision)
+{
+    const PatternHost& h = p->h;
+    const PatternLayout& L = p->L;
+    const int64_t n = h.n, nnz = h.nnz, E = h.n_elem, G = grid_for(n);
+    auto b = [](int64_t count, int64_t size) { return std::max<int64_t>(count, 1) * size; };
+    const int ed = edim_of(h.etype);
+    int64_t t = b(n + 1, 4) + b(nnz, 4) + b(n, 4) + b((int64_t)h.conn.size(), 4) +
+                b((int64_t)h.color_elems.size(), 4) + b((int64_t)h.slot.size(), 4) + b(nnz, 8) +
+                b(n, 8) + 12 * b(n, 8) + b(16 * G, 8) + b(8, 8) + b(1, sizeof(KrylovState)) +
+                b(n, 1) + b(n, 8) + b(1, sizeof(SysDev)) + b(1, sizeof(KrylovParams));
+    if (precision == OGSX_FP32_IR) t += b(nnz, 4) + 12 * b(n, 4) + b(n, 8) + b(1, sizeof(IRState));
+    t += b((int64_t)L.trow.size(), 4) + b((int64_t)L.tcb.size(), 4) +
+         (L.tile16 ? b((int64_t)L.tc16.size(), 2) : 0);
+    t += b((int64_t)L.sptr.size(), 4) + b((int64_t)L.sw.size(), 4) + b((int64_t)L.scb.size(), 4) +
+         b((int64_t)L.sc32.size(), 4) + b((int64_t)L.ssrc.size(), 4) + b(L.npad, 8) +
+         (precision == OGSX_FP32_IR ? b(L.npad, 4) : 0) + (L.sell16 ? b((int64_t)L.sc16.size(), 2) : 0);
+    // element data
+    t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
+         b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
+    return t;
 }
 
 // ---- profiling helpers ----------------------------------------------------
@@ -488,6 +553,7 @@ int ogsx_pattern_create(int64_t n_nodes, int64_t n_elem, int etype, const int64_
         delete p;
         return fail(rc, msg);
     }
+    build_layout(p->h, p->L);
     *out = p;
     return OGSX_OK;
 }
@@ -517,6 +583,21 @@ int ogsx_pattern_colors(const ogsx_pattern* p, int32_t* color)
     return OGSX_OK;
 }
 
+int ogsx_pattern_footprint(const ogsx_pattern* p, int precision, int64_t* bytes)
+{
+    if (!p || !bytes) return fail(OGSX_EINVAL, "null argument");
+    if (precision != OGSX_FP64 && precision != OGSX_FP32_IR) return fail(OGSX_EINVAL, "unknown precision");
+    *bytes = system_bytes(p, precision);
+    return OGSX_OK;
+}
+
+int ogsx_system_bytes(const ogsx_system* s, int64_t* bytes)
+{
+    if (!s || !bytes) return fail(OGSX_EINVAL, "null argument");
+    *bytes = s->bytes;
+    return OGSX_OK;
+}
+
 int ogsx_pattern_destroy(ogsx_pattern* p)
 {
     delete p;
@@ -532,6 +613,7 @@ int ogsx_system_destroy(ogsx_system* s)
     if (s->ready) (void)hipEventDestroy(s->ready);
     for (void* q : s->allocs) (void)hipFree(q);
     if (s->prm_host) (void)hipHostFree(s->prm_host);
+    if (s->irs_host) (void)hipHostFree(s->irs_host);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -562,10 +644,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(OGSX_EHIP, "hipStreamCreate failed"));
     const PatternHost& h = p->h;
-    std::vector<int> ptr32(h.n + 1), col32(h.nnz), dg32(h.n);
-    for (int64_t i = 0; i <= h.n; i++) ptr32[i] = (int)h.ptr[i];
-    for (int64_t k = 0; k < h.nnz; k++) col32[k] = (int)h.col[k];
-    for (int64_t i = 0; i < h.n; i++) dg32[i] = (int)h.diag[i];
+    const PatternLayout& Ly = p->L;
     if ((rc = dalloc(s, &s->ptr, h.n + 1)) || (rc = dalloc(s, &s->col, h.nnz)) ||
         (rc = dalloc(s, &s->diagpos, h.n)) || (rc = dalloc(s, &s->conn, h.conn.size())) ||
         (rc = dalloc(s, &s->color_elems, h.color_elems.size())) ||
@@ -583,12 +662,17 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     if ((rc = dalloc(s, &s->dev, 1)) || (rc = dalloc(s, &s->prm, 1))) return cleanup(rc);
     if (hipHostMalloc((void**)&s->prm_host, sizeof(KrylovParams)) != hipSuccess)
         return cleanup(fail(OGSX_ENOMEM, "hipHostMalloc failed"));
+    if (precision == OGSX_FP32_IR) {
+        if ((rc = dalloc(s, &s->irs, 1))) return cleanup(rc);
+        if (hipHostMalloc((void**)&s->irs_host, sizeof(IRState)) != hipSuccess)
+            return cleanup(fail(OGSX_ENOMEM, "hipHostMalloc failed"));
+    }
     std::memset(s->prm_host, 0, sizeof(KrylovParams));
     auto up = [&](void* d, const void* src, size_t bytes) {
         return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s->stream);
     };
-    if (up(s->ptr, ptr32.data(), 4 * (h.n + 1)) || up(s->col, col32.data(), 4 * h.nnz) ||
-        up(s->diagpos, dg32.data(), 4 * h.n) ||
+    if (up(s->ptr, Ly.ptr32.data(), 4 * (h.n + 1)) || up(s->col, Ly.col32.data(), 4 * h.nnz) ||
+        up(s->diagpos, Ly.dg32.data(), 4 * h.n) ||
         up(s->conn, h.conn.data(), 4 * h.conn.size()) ||
         up(s->color_elems, h.color_elems.data(), 4 * h.color_elems.size()) ||
         up(s->slot, h.slot.data(), 4 * h.slot.size()))
@@ -601,40 +685,30 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess)
         return cleanup(fail(OGSX_EHIP, "initialisation kernels failed"));
     {
-        std::vector<int> trow, tcb;
-        std::vector<uint16_t> c16;
-        build_tiles(h, trow, tcb, c16, s->tile16);
-        s->n_tiles = (int)trow.size() - 1;
-        if ((rc = dalloc(s, &s->tile_row, trow.size())) || (rc = dalloc(s, &s->tile_cb, tcb.size())))
+        s->tile16 = Ly.tile16;
+        s->n_tiles = (int)Ly.trow.size() - 1;
+        if ((rc = dalloc(s, &s->tile_row, Ly.trow.size())) || (rc = dalloc(s, &s->tile_cb, Ly.tcb.size())))
             return cleanup(rc);
-        if (hipMemcpyAsync(s->tile_row, trow.data(), 4 * trow.size(), hipMemcpyHostToDevice, s->stream) ||
-            hipMemcpyAsync(s->tile_cb, tcb.data(), 4 * tcb.size(), hipMemcpyHostToDevice, s->stream))
+        if (s->tile16 && (rc = dalloc(s, &s->col16, Ly.tc16.size()))) return cleanup(rc);
+        if (up(s->tile_row, Ly.trow.data(), 4 * Ly.trow.size()) ||
+            up(s->tile_cb, Ly.tcb.data(), 4 * Ly.tcb.size()) ||
+            (s->tile16 && up(s->col16, Ly.tc16.data(), 2 * Ly.tc16.size())))
             return cleanup(fail(OGSX_EHIP, "upload of the tiles failed"));
-        if (s->tile16) {
-            if ((rc = dalloc(s, &s->col16, c16.size()))) return cleanup(rc);
-            if (hipMemcpyAsync(s->col16, c16.data(), 2 * c16.size(), hipMemcpyHostToDevice, s->stream) ||
-                hipStreamSynchronize(s->stream))
-                return cleanup(fail(OGSX_EHIP, "upload of the 16-bit columns failed"));
-        }
-        if (hipStreamSynchronize(s->stream) != hipSuccess)
-            return cleanup(fail(OGSX_EHIP, "tile upload failed"));
     }
     {
-        std::vector<int> sptr, sw, scb, c32, src;
-        std::vector<uint16_t> c16;
-        build_sell(h, sptr, sw, scb, c32, c16, src, s->sell16);
-        s->n_slices = (int)sw.size();
-        s->npad = sptr.back();
-        if ((rc = dalloc(s, &s->sell_ptr, sptr.size())) || (rc = dalloc(s, &s->sell_w, sw.size())) ||
-            (rc = dalloc(s, &s->sell_cb, scb.size())) || (rc = dalloc(s, &s->sell_c32, c32.size())) ||
-            (rc = dalloc(s, &s->sell_src, src.size())) || (rc = dalloc(s, &s->sval64, s->npad)))
+        s->sell16 = Ly.sell16;
+        s->n_slices = (int)Ly.sw.size();
+        s->npad = Ly.npad;
+        if ((rc = dalloc(s, &s->sell_ptr, Ly.sptr.size())) || (rc = dalloc(s, &s->sell_w, Ly.sw.size())) ||
+            (rc = dalloc(s, &s->sell_cb, Ly.scb.size())) || (rc = dalloc(s, &s->sell_c32, Ly.sc32.size())) ||
+            (rc = dalloc(s, &s->sell_src, Ly.ssrc.size())) || (rc = dalloc(s, &s->sval64, s->npad)))
             return cleanup(rc);
         if (precision == OGSX_FP32_IR && (rc = dalloc(s, &s->sval32, s->npad))) return cleanup(rc);
-        if (s->sell16 && (rc = dalloc(s, &s->sell_c16, c16.size()))) return cleanup(rc);
-        if (up(s->sell_ptr, sptr.data(), 4 * sptr.size()) || up(s->sell_w, sw.data(), 4 * sw.size()) ||
-            up(s->sell_cb, scb.data(), 4 * scb.size()) || up(s->sell_c32, c32.data(), 4 * c32.size()) ||
-            up(s->sell_src, src.data(), 4 * src.size()) ||
-            (s->sell16 && up(s->sell_c16, c16.data(), 2 * c16.size())) ||
+        if (s->sell16 && (rc = dalloc(s, &s->sell_c16, Ly.sc16.size()))) return cleanup(rc);
+        if (up(s->sell_ptr, Ly.sptr.data(), 4 * Ly.sptr.size()) || up(s->sell_w, Ly.sw.data(), 4 * Ly.sw.size()) ||
+            up(s->sell_cb, Ly.scb.data(), 4 * Ly.scb.size()) || up(s->sell_c32, Ly.sc32.data(), 4 * Ly.sc32.size()) ||
+            up(s->sell_src, Ly.ssrc.data(), 4 * Ly.ssrc.size()) ||
+            (s->sell16 && up(s->sell_c16, Ly.sc16.data(), 2 * Ly.sc16.size())) ||
             hipStreamSynchronize(s->stream))
             return cleanup(fail(OGSX_EHIP, "upload of the SELL layout failed"));
     }
@@ -655,6 +729,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
         d.red = s->red;
         d.st = s->st;
         d.prm = s->prm;
+        d.irs = s->irs;
         d.n = s->n;
         d.G = s->G;
         d.nnz = (int)s->nnz;
@@ -955,7 +1030,6 @@ int ogsx_solve(ogsx_system* s, const ogsx_solver_config* cfg, ogsx_solve_info* i
     if (!s->solo) OK_TRY(make_set({s}, s->stream, &s->solo));
     MemberSet* m = s->solo;
     OK_TRY(prepare_member(s, cfg, s->stream));
-    OK_TRY(set_full_maps(m));
     OK_TRY(set_start(m, {0}));
     OK_TRY(set_advance(m, INT_MAX));
     HIP_TRY(hipGetLastError());
@@ -1074,7 +1148,6 @@ int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
         HIP_TRY(hipEventRecord(s->ready, s->stream));
         HIP_TRY(hipStreamWaitEvent(g->stream, s->ready, 0));
     }
-    OK_TRY(set_full_maps(m));
     return set_start(m, which);
 }
 

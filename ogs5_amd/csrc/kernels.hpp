@@ -689,7 +689,9 @@ struct KVec {
     T* sh;
 };
 
-// Per-solve parameters of one system, written by the host before a solve.
+// Per-solve parameters of one system.  The recurrence fields are written by
+// the host (FP64 solves) or by k_ir_decide (the FP32 recurrence of IR); the
+// ir_* fields by the host when an IR solve starts.
 struct KrylovParams {
     double tol;         // stopping test ||r||/||b|| < tol
     double bnorm_prev;  // CheckNormRHS reference (0 switches the test off)
@@ -698,6 +700,27 @@ struct KrylovParams {
     double unscale;     // IR: x64 += unscale * x32
     int max_iter;
     int jacobi;         // 1 Jacobi, 0 no preconditioner
+    double ir_tol;        // FP64 stopping test of the refinement
+    double ir_bnorm_prev; // CheckNormRHS reference of the refinement
+    double ir_delta;      // restart: FP32 solve tolerance; reliable: update drop
+    int ir_max_iter;
+    int ir_max_outer;     // restart: refinement steps
+    int ir_reliable;      // 1 reliable-update CG, 0 restarted refinement
+    int pad;
+};
+
+// Refinement state of an IR solve, advanced by k_ir_decide.
+struct IRState {
+    double bnorm;   // FP64 ||b||
+    double error;   // FP64 ||b - A x|| / ||b|| at the last residual step
+    double maxr;    // reliable updates: scaled ||r|| at the last update
+    int started;    // an FP32 recurrence has run
+    int status;     // -1 running; else finished: 1 converged, 0 max_iter,
+                    // 2 skipped by CheckNormRHS, 3 breakdown
+    int action;     // of the last decision: 0 none, 1 new recurrence, 2 resume
+    int total;      // Krylov iterations so far
+    int updates;    // residual steps after the first
+    int pad;
 };
 
 struct SysDev {
@@ -715,7 +738,8 @@ struct SysDev {
     double* part;        // partial sums [16][G]
     double* red;         // reduced scalars
     KrylovState* st;
-    const KrylovParams* prm;
+    KrylovParams* prm;
+    IRState* irs;        // IR systems
     int n, G, nnz, npad;
 };
 
@@ -802,11 +826,19 @@ __global__ __launch_bounds__(kBlock) void k_cast_f32_g(Grp g)
 // ---- Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
 
 // s = A x; r = b - s; z = r/d; p = z; partials (b,b), (r,r), (r,z)
+// IR event kernels run only for the systems whose last k_ir_decide chose
+// `action` (1 new recurrence, 2 resume); FP64 solves pass want = 0.
+__device__ __forceinline__ bool skip_action(const SysDev& S, int want)
+{
+    return want != 0 && S.irs->action != want;
+}
+
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_init(Grp g)
+__global__ __launch_bounds__(kBlock) void k_cg_init(Grp g, int want)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
+    if (skip_action(S, want)) return;
     const MatView<VT> m = mv_of<VT>(S);
     const KVec<T> v = kv_of<T>(S);
     double acc[3] = {0.0, 0.0, 0.0};
@@ -823,9 +855,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Grp g)
 }
 
 // One block per system: CheckNormRHS (:1104), initial residual test, rr.
-__global__ void k_cg_init_final(Grp g)
+__global__ void k_cg_init_final(Grp g, int want)
 {
     const SysDev& S = *sys_of_block(g);
+    if (skip_action(S, want)) return;
     double s[3];
     reduce_partials<3>(S.part, S.G, s);
     if (threadIdx.x != 0) return;
@@ -993,6 +1026,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_rescan(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
+    if (skip_action(S, 2)) return;
     const KVec<T> v = kv_of<T>(S);
     double acc[2] = {0.0, 0.0};
     vec_chunk(S.n, c.G, c.b, [&](int i) {
@@ -1011,6 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_resume(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
+    if (skip_action(S, 2)) return;
     double s[2];
     reduce_partials<2>(S.part + 4 * (size_t)c.G, c.G, s);
     KrylovState* st = S.st;
@@ -1029,10 +1064,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_resume(Grp g)
 
 // s = A x; r0 = b - s; r = r0; v = p = 0; partials (r,r), (r0,r), (b,b)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_init(Grp g)
+__global__ __launch_bounds__(kBlock) void k_bi_init(Grp g, int want)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
+    if (skip_action(S, want)) return;
     const MatView<VT> m = mv_of<VT>(S);
     const KVec<T> v = kv_of<T>(S);
     double acc[3] = {0.0, 0.0, 0.0};
@@ -1049,9 +1085,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(Grp g)
     store_partials<3>(acc, S.part, c.G, c.b);
 }
 
-__global__ void k_bi_init_final(Grp g)
+__global__ void k_bi_init_final(Grp g, int want)
 {
     const SysDev& S = *sys_of_block(g);
+    if (skip_action(S, want)) return;
     double s[3];
     reduce_partials<3>(S.part, S.G, s);
     if (threadIdx.x != 0) return;
@@ -1260,26 +1297,111 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
     store_partials<2>(acc, S.part, c.G, c.b);
 }
 
-// r32 (into b32 when into_b, else r32) = scale * r64; x32 = 0
-__global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g, int into_b)
+// x64 += unscale * x32: folds the FP32 correction of the recurrence that
+// just stopped into the FP64 iterate.
+__global__ __launch_bounds__(kBlock) void k_ir_fold(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
+    if (!S.irs->started) return;
+    const double u = S.prm->unscale;
+    vec_chunk(S.n, c.G, c.b, [&](int i) { S.v64.x[i] += u * (double)S.v32.x[i]; });
+}
+
+// One block per listed system: after k_ir_residual, the refinement step.
+// First call of a solve: ||b||, CheckNormRHS (equation_class.cpp:1104).
+// Then the FP64 stopping test, the iteration / refinement-step limits, and
+// the choice of the next FP32 recurrence: a new one on r / ||r|| (restarted
+// refinement; reliable updates' first) or the resumption of the running one
+// from the true residual (reliable updates, Clark et al., Comput. Phys.
+// Commun. 181 (2010) 1517).
+__global__ void k_ir_decide(Grp g)
+{
+    const SysDev& S = *sys_of_block(g);
+    double s[2];
+    reduce_partials<2>(S.part, S.G, s);
+    if (threadIdx.x != 0) return;
+    IRState& R = *S.irs;
+    KrylovParams& P = *S.prm;
+    KrylovState& st = *S.st;
+    R.action = 0;
+    // a finished solve leaves its recurrence stopped, so batched launches skip it
+    auto finish = [&](int status) {
+        R.status = status;
+        if (st.done == 0) st.done = 4;
+    };
+    const double rn = sqrt(s[1]);
+    int inner_status = 0, inner_iters = 0;
+    if (!R.started) {
+        R.bnorm = sqrt(s[0]);
+        R.total = 0;
+        R.updates = 0;
+        if (P.ir_bnorm_prev > 0.0 && R.bnorm / P.ir_bnorm_prev < P.ir_tol) {
+            R.error = R.bnorm / P.ir_bnorm_prev;
+            finish(2);
+            return;
+        }
+        if (R.bnorm < DBL_MIN) {
+            R.error = 0.0;
+            finish(2);
+            return;
+        }
+    } else {
+        inner_iters = st.iter;
+        inner_status = st.done == 2 ? 2 : (st.done == 3 || st.done == 6) ? 3 : 1;
+        R.total = P.ir_reliable ? st.iter : R.total + max(st.iter, 1);
+        R.updates++;
+    }
+    R.error = rn / R.bnorm;
+    if (R.error < P.ir_tol) {
+        finish(1);
+        return;
+    }
+    if (R.total >= P.ir_max_iter || (!P.ir_reliable && R.updates >= P.ir_max_outer) ||
+        inner_status == 2) {
+        finish(0);
+        return;
+    }
+    if (inner_status == 3 && inner_iters == 0) {
+        finish(3);
+        return;
+    }
+    if (P.ir_reliable && R.started) {
+        R.maxr = rn * P.scale;
+        P.upd_thresh = P.ir_delta * R.maxr;
+        R.action = 2;
+    } else {
+        P.scale = 1.0 / rn;
+        P.unscale = rn;
+        P.bnorm_prev = 0.0;
+        if (P.ir_reliable) {
+            P.tol = P.ir_tol * R.bnorm * P.scale;  // the FP64 target in scaled units
+            R.maxr = 1.0;
+            P.upd_thresh = P.ir_delta;
+        } else {
+            P.tol = P.ir_delta;
+            P.upd_thresh = 0.0;
+        }
+        P.max_iter = max(1, P.ir_max_iter - R.total);
+        R.action = 1;
+    }
+    R.started = 1;
+}
+
+// The decided recurrence's start vectors: new recurrence b32 = scale r64,
+// resumption r32 = scale r64; x32 = 0 either way.
+__global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const int act = S.irs->action;
+    if (act == 0) return;
     const double scale = S.prm->scale;
-    float* dst = into_b ? S.v32.b : S.v32.r;
+    float* dst = act == 1 ? S.v32.b : S.v32.r;
     vec_chunk(S.n, c.G, c.b, [&](int i) {
         dst[i] = (float)(S.r64[i] * scale);
         S.v32.x[i] = 0.0f;
     });
-}
-
-// x64 += unscale * x32
-__global__ __launch_bounds__(kBlock) void k_ir_correct(Grp g)
-{
-    const Ctx c = ctx_of(g);
-    const SysDev& S = *c.S;
-    const double u = S.prm->unscale;
-    vec_chunk(S.n, c.G, c.b, [&](int i) { S.v64.x[i] += u * (double)S.v32.x[i]; });
 }
 
 // ------------------------------------------------------------------------

@@ -161,6 +161,67 @@ def ensemble(count: int = 256, *, side_range=(64, 512), sigma_range=(0.5, 2.5),
             for i, (s, sg) in enumerate(zip(sides, sigmas))]
 
 
+@dataclass
+class RequestSpec:
+    """One served solve request: a flow scenario drawn from the request mix."""
+    rid: int
+    kind: str        # "quad" (2-D) or "hex" (3-D)
+    side: int        # cells per axis
+    steps: int       # implicit time steps
+    seed: int
+    sigma_ln_k: float
+    arrival: float   # seconds after the stream starts
+
+
+def request_stream(count: int, rate: float = 200.0, *, seed: int = 0,
+                   quad_sides=(32, 512), hex_sides=(16, 96), steps=(1, 20),
+                   sigma_range=(0.5, 2.5)) -> List[RequestSpec]:
+    """Poisson arrivals at `rate` per second; each request is a 2-D quad or a
+    3-D hex flow problem (even odds) with log-uniform side length, its own
+    seeded lognormal K, 1-20 steps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    out = []
+    for i in range(count):
+        kind = "quad" if rng.random() < 0.5 else "hex"
+        lo, hi = quad_sides if kind == "quad" else hex_sides
+        side = int(np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        out.append(RequestSpec(i, kind, side, int(rng.integers(steps[0], steps[1] + 1)),
+                               1000 + i, float(rng.uniform(*sigma_range)), float(t[i])))
+    return out
+
+
+def request_scenario(spec: RequestSpec) -> Scenario:
+    """The flow problem of a request: Dirichlet heads on a random pair of
+    opposite faces (random values), one sink at a random interior node,
+    implicit Euler from h = 0 (2-D: 1000 m square, 50 m correlation, dt
+    3600 s; 3-D: unit cells, 8-cell correlation, dt 100 s)."""
+    rng = np.random.Generator(np.random.PCG64(spec.seed))
+    n = spec.side
+    if spec.kind == "quad":
+        m = quad_mesh(n, n, 1000.0, 1000.0)
+        K = lognormal_conductivity(m, np.log(1e-4), spec.sigma_ln_k, 50.0, spec.seed)
+        length, dt, q = 1000.0, 3600.0, -0.01
+        axes = 2
+    else:
+        m = hex_mesh(n, n, n)
+        K = lognormal_conductivity(m, np.log(1e-4), spec.sigma_ln_k, 8.0, spec.seed)
+        length, dt, q = float(n), 100.0, -1e-4
+        axes = 3
+    axis = int(rng.integers(0, axes))
+    lo_face = m.nodes_where(axis, 0.0)
+    hi_face = m.nodes_where(axis, length)
+    h_lo, h_hi = rng.uniform(0.0, 10.0, 2)
+    bc_idx = np.concatenate([lo_face, hi_face]).astype(np.int64)
+    bc_val = np.concatenate([np.full(lo_face.size, h_lo), np.full(hi_face.size, h_hi)])
+    ijk = [int(rng.integers(1, n)) for _ in range(axes)]
+    sink = m.node_index(*ijk)
+    return Scenario(f"request_{spec.rid}_{spec.kind}{n}", m, L.LIQUID_FLOW,
+                    np.full(m.n_elem, 1e-4), K, 1, bc_idx, bc_val, np.array([sink], np.int64),
+                    np.array([q]), np.zeros(m.n_nodes), [Step(dt, 1.0)] * spec.steps,
+                    meta=dict(seed=spec.seed, sigma_ln_k=spec.sigma_ln_k))
+
+
 def bmskel() -> Scenario:
     """The reference's tests/data/bmskel deck: one line element of length 1,
     HEAT_TRANSPORT, T0 = 273, Dirichlet 274 at POINT0 (node 0), one step of
