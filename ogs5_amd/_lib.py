@@ -161,11 +161,11 @@ def _i64(a):
 def solver_config(method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
                   inner_tol=None, max_outer=50, ir_mode=None) -> SolverConfig:
     """ogsx_solver_config; for IR systems CG defaults to reliable updates
-    (inner_tol = delta 0.03), BiCGStab to restarted refinement (1e-4)."""
+    (inner_tol = delta 0.1), BiCGStab to restarted refinement (1e-4)."""
     if ir_mode is None:
         ir_mode = IR_RELIABLE if method == CG else IR_RESTART
     if inner_tol is None:
-        inner_tol = 0.03 if ir_mode == IR_RELIABLE else 1e-4
+        inner_tol = 0.1 if ir_mode == IR_RELIABLE else 1e-4
     return SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer, ir_mode)
 
 
@@ -260,7 +260,7 @@ class System:
 
     def solve(self, method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
               inner_tol=None, max_outer=50, ir_mode=None) -> dict:
-        """IR systems: CG defaults to reliable updates (inner_tol = delta 0.03),
+        """IR systems: CG defaults to reliable updates (inner_tol = delta 0.1),
         BiCGStab to restarted refinement (inner_tol 1e-4)."""
         cfg = solver_config(method, precond, tol, max_iter, inner_tol, max_outer, ir_mode)
         info = SolveInfo()

@@ -291,7 +291,7 @@ void build_sell(const PatternHost& h, std::vector<int>& sptr, std::vector<int>& 
 // Brings the SELL value copies up to date with A (after assembly / BC).
 int sync_sell(ogsx_system* s, bool want64, bool want32)
 {
-    const Grp g1{s->dev, nullptr, nullptr};
+    const Grp g1{s->dev, nullptr, nullptr, 1};
     if (want64 && s->sval64_version != s->a_version) {
         k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1);
         s->sval64_version = s->a_version;
@@ -1344,7 +1344,7 @@ int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* 
         (void)hipEventRecord(e1, s->stream);
     } else {
         if (!is_sell(s->fmt32))
-            k_cast_f32_g<<<s->G, kBlock, 0, s->stream>>>(Grp{s->dev, nullptr, nullptr});
+            k_cast_f32_g<<<s->G, kBlock, 0, s->stream>>>(Grp{s->dev, nullptr, nullptr, 1});
         OK_TRY(sync_sell(s, false, is_sell(s->fmt32)));
         const MatView<float> mv = mat_view(s, (const float*)s->A32);
         (void)hipEventRecord(e0, s->stream);

@@ -43,7 +43,7 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Block-reduces NV values and stores them as partials[k*G + blockIdx.x].
+// Block-reduces NV values and stores them as partials[k*G + b].
 template <int NV>
 __device__ __forceinline__ void store_partials(double (&v)[NV], double* partials, int G, int b)
 {
@@ -64,6 +64,7 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* partials
             partials[(size_t)k * G + b] = s;
         }
     }
+    __syncthreads();  // sm is reused by the block's next virtual block
 }
 
 // Sums G partials per value in a fixed order; every thread gets the result.
@@ -756,31 +757,52 @@ __device__ __forceinline__ const KVec<double>& kv_of<double>(const SysDev& S) { 
 template <>
 __device__ __forceinline__ const KVec<float>& kv_of<float>(const SysDev& S) { return S.v32; }
 
-// Launch geometry: blk2sys = nullptr for one system (grid = its G blocks);
-// otherwise block -> index into sys, blk0[i] = first block of system i.
-// Kernels with one block per system use blk2sys as the block -> system list.
+// Launch geometry.  A system's rows are split into G chunks ("virtual
+// blocks", each with its own partial sums); a launched block works on vpb
+// consecutive virtual blocks of one system, one after the other, with the
+// same thread -> row mapping and reduction order as if each were a block of
+// its own.  So vpb changes the number of launched blocks, never a result.
+// blk2sys = nullptr: one system, grid = ceil(G / vpb).  Otherwise blk2sys
+// maps a launched block to its system and blk0[i] is system i's first
+// launched block.  One-block-per-system kernels use blk2sys as the list.
 struct Grp {
     const SysDev* sys;
     const int* blk2sys;
     const int* blk0;
+    int vpb;
 };
 
 struct Ctx {
     const SysDev* S;
-    int G, b;
+    int G, b0, nb;  // virtual blocks [b0, b0 + nb) of the system's G
 };
 
 __device__ __forceinline__ Ctx ctx_of(const Grp& g)
 {
-    if (!g.blk2sys) return Ctx{g.sys, (int)gridDim.x, (int)blockIdx.x};
-    const int si = g.blk2sys[blockIdx.x];
-    const SysDev* S = g.sys + si;
-    return Ctx{S, S->G, (int)blockIdx.x - g.blk0[si]};
+    const SysDev* S;
+    int first;
+    if (!g.blk2sys) {
+        S = g.sys;
+        first = (int)blockIdx.x * g.vpb;
+    } else {
+        const int si = g.blk2sys[blockIdx.x];
+        S = g.sys + si;
+        first = ((int)blockIdx.x - g.blk0[si]) * g.vpb;
+    }
+    const int G = S->G;
+    return Ctx{S, G, first, min(g.vpb, G - first)};
 }
 
 __device__ __forceinline__ const SysDev* sys_of_block(const Grp& g)
 {
     return g.sys + (g.blk2sys ? g.blk2sys[blockIdx.x] : (int)blockIdx.x);
+}
+
+// Vector body over the rows of the block's virtual blocks.
+template <typename Body>
+__device__ __forceinline__ void vec_vblocks(const Ctx& c, int n, Body&& body)
+{
+    for (int v = 0; v < c.nb; v++) vec_chunk(n, c.G, c.b0 + v, body);
 }
 
 // Jacobi diagonal: Precond_Jacobi's a_ii with |a_ii| < DBL_MIN -> 1
@@ -792,7 +814,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(Grp g)
     const SysDev& S = *c.S;
     T* d = kv_of<T>(S).d;
     const int jac = S.prm->jacobi;
-    vec_chunk(S.n, c.G, c.b, [&](int i) {
+    vec_vblocks(c, S.n, [&](int i) {
         double v = (jac && S.diagpos[i] >= 0) ? S.A[S.diagpos[i]] : 1.0;
         if (fabs(v) < DBL_MIN) v = 1.0;
         d[i] = (T)v;
@@ -810,7 +832,7 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g)
         sv = S.sval64;
     else
         sv = S.sval32;
-    vec_chunk(S.npad, c.G, c.b, [&](int p) {
+    vec_vblocks(c, S.npad, [&](int p) {
         const int k = S.sell_src[p];
         sv[p] = k >= 0 ? (VT)S.A[k] : (VT)0;
     });
@@ -820,12 +842,9 @@ __global__ __launch_bounds__(kBlock) void k_cast_f32_g(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    vec_chunk(S.nnz, c.G, c.b, [&](int k) { S.A32[k] = (float)S.A[k]; });
+    vec_vblocks(c, S.nnz, [&](int k) { S.A32[k] = (float)S.A[k]; });
 }
 
-// ---- Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
-
-// s = A x; r = b - s; z = r/d; p = z; partials (b,b), (r,r), (r,z)
 // IR event kernels run only for the systems whose last k_ir_decide chose
 // `action` (1 new recurrence, 2 resume); FP64 solves pass want = 0.
 __device__ __forceinline__ bool skip_action(const SysDev& S, int want)
@@ -833,6 +852,9 @@ __device__ __forceinline__ bool skip_action(const SysDev& S, int want)
     return want != 0 && S.irs->action != want;
 }
 
+// ---- Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
+
+// s = A x; r = b - s; z = r/d; p = z; partials (b,b), (r,r), (r,z)
 template <int FMT, typename VT, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init(Grp g, int want)
 {
@@ -841,17 +863,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Grp g, int want)
     if (skip_action(S, want)) return;
     const MatView<VT> m = mv_of<VT>(S);
     const KVec<T> v = kv_of<T>(S);
-    double acc[3] = {0.0, 0.0, 0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.x, c.G, c.b, [&](int i, T ax) {
-        const T ri = v.b[i] - ax;
-        const T zi = ri / v.d[i];
-        v.r[i] = ri;
-        v.p[i] = zi;
-        acc[0] += (double)v.b[i] * (double)v.b[i];
-        acc[1] += (double)ri * (double)ri;
-        acc[2] += (double)ri * (double)zi;
-    });
-    store_partials<3>(acc, S.part, c.G, c.b);
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[3] = {0.0, 0.0, 0.0};
+        spmv_rows<FMT, VT, T, T>(m, v.x, c.G, b, [&](int i, T ax) {
+            const T ri = v.b[i] - ax;
+            const T zi = ri / v.d[i];
+            v.r[i] = ri;
+            v.p[i] = zi;
+            acc[0] += (double)v.b[i] * (double)v.b[i];
+            acc[1] += (double)ri * (double)ri;
+            acc[2] += (double)ri * (double)zi;
+        });
+        store_partials<3>(acc, S.part, c.G, b);
+    }
 }
 
 // One block per system: CheckNormRHS (:1104), initial residual test, rr.
@@ -868,6 +893,7 @@ __global__ void k_cg_init_final(Grp g, int want)
     st->iter = 0;
     st->max_iter = P.max_iter;
     st->done = 0;
+    st->pend = 0;
     if (P.bnorm_prev > 0.0 && bnorm_new / P.bnorm_prev < P.tol) {
         st->error = bnorm_new / P.bnorm_prev;
         st->bnorm = bnorm_new;
@@ -898,12 +924,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
     if (S.st->done) return;
     const MatView<VT> m = mv_of<VT>(S);
     const KVec<T> v = kv_of<T>(S);
-    double acc[1] = {0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.p, c.G, c.b, [&](int i, T ax) {
-        v.s[i] = ax;
-        acc[0] += (double)v.p[i] * (double)ax;
-    });
-    store_partials<1>(acc, S.part, c.G, c.b);
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[1] = {0.0};
+        spmv_rows<FMT, VT, T, T>(m, v.p, c.G, b, [&](int i, T ax) {
+            v.s[i] = ax;
+            acc[0] += (double)v.p[i] * (double)ax;
+        });
+        store_partials<1>(acc, S.part, c.G, b);
+    }
 }
 
 // K2: alpha = rr/(p,s); x += alpha p; r -= alpha s; partials (r,r), (r/d, r)
@@ -918,49 +947,52 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
     reduce_partials<1>(S.part, c.G, ps);
     const double rr = st->rr;
     const T alpha = (T)(rr / ps[0]);
-    double acc[2] = {0.0, 0.0};
     const KVec<T> v = kv_of<T>(S);
     T* __restrict__ x = v.x;
     T* __restrict__ r = v.r;
     const T* __restrict__ p = v.p;
     const T* __restrict__ sv = v.s;
     const T* __restrict__ d = v.d;
-    int beg, end;
-    chunk_of(S.n, c.G, c.b, beg, end);
-    constexpr int U = 4;
-    int i = beg + threadIdx.x;
-    for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
-        T xi[U], pi[U], ri[U], si[U], di[U];
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[2] = {0.0, 0.0};
+        int beg, end;
+        chunk_of(S.n, c.G, b, beg, end);
+        constexpr int U = 4;
+        int i = beg + threadIdx.x;
+        for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
+            T xi[U], pi[U], ri[U], si[U], di[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = i + u * kBlock;
-            xi[u] = x[k];
-            pi[u] = p[k];
-            ri[u] = r[k];
-            si[u] = sv[k];
-            di[u] = d[k];
+            for (int u = 0; u < U; u++) {
+                const int k = i + u * kBlock;
+                xi[u] = x[k];
+                pi[u] = p[k];
+                ri[u] = r[k];
+                si[u] = sv[k];
+                di[u] = d[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = i + u * kBlock;
+                x[k] = xi[u] + alpha * pi[u];
+                const T rn = ri[u] - alpha * si[u];
+                r[k] = rn;
+                const T zi = rn / di[u];
+                acc[0] += (double)rn * (double)rn;
+                acc[1] += (double)zi * (double)rn;
+            }
         }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = i + u * kBlock;
-            x[k] = xi[u] + alpha * pi[u];
-            const T rn = ri[u] - alpha * si[u];
-            r[k] = rn;
-            const T zi = rn / di[u];
+        for (; i < end; i += kBlock) {
+            x[i] += alpha * p[i];
+            const T rn = r[i] - alpha * sv[i];
+            r[i] = rn;
+            const T zi = rn / d[i];
             acc[0] += (double)rn * (double)rn;
             acc[1] += (double)zi * (double)rn;
         }
+        store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, b);
     }
-    for (; i < end; i += kBlock) {
-        x[i] += alpha * p[i];
-        const T rn = r[i] - alpha * sv[i];
-        r[i] = rn;
-        const T zi = rn / d[i];
-        acc[0] += (double)rn * (double)rn;
-        acc[1] += (double)zi * (double)rn;
-    }
-    store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, c.b);
-    if (c.b == 0 && threadIdx.x == 0) {
+    if (c.b0 == 0 && threadIdx.x == 0) {
         st->rr_old = rr;
         st->alpha = rr / ps[0];
     }
@@ -968,7 +1000,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
 
 // K3: error test; beta = (z,r)/rr_old; p = r/d + beta p.  With reliable
 // updates (upd_thresh > 0) the recurrence stops with done = 5 once ||r|| <
-// upd_thresh, leaving p for the resumed iteration.
+// upd_thresh, leaving p for the resumed iteration.  Every block takes the
+// same branch (same reduced values), so a stop leaves p untouched everywhere.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
 {
@@ -982,7 +1015,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
     const double error = rnorm / st->bnorm;
     const double tol = S.prm->tol;
     if (error < tol || rnorm < S.prm->upd_thresh) {
-        if (c.b == 0 && threadIdx.x == 0) {
+        if (c.b0 == 0 && threadIdx.x == 0) {
             st->error = error;
             st->iter = st->iter + 1;
             st->done = error < tol ? 1 : 5;
@@ -994,24 +1027,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
     T* __restrict__ p = v.p;
     const T* __restrict__ r = v.r;
     const T* __restrict__ d = v.d;
-    int beg, end;
-    chunk_of(S.n, c.G, c.b, beg, end);
-    constexpr int U = 4;
-    int i = beg + threadIdx.x;
-    for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
-        T pi[U], ri[U], di[U];
+    for (int vb = 0; vb < c.nb; vb++) {
+        int beg, end;
+        chunk_of(S.n, c.G, c.b0 + vb, beg, end);
+        constexpr int U = 4;
+        int i = beg + threadIdx.x;
+        for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
+            T pi[U], ri[U], di[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = i + u * kBlock;
-            pi[u] = p[k];
-            ri[u] = r[k];
-            di[u] = d[k];
+            for (int u = 0; u < U; u++) {
+                const int k = i + u * kBlock;
+                pi[u] = p[k];
+                ri[u] = r[k];
+                di[u] = d[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
         }
-#pragma unroll
-        for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
+        for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
     }
-    for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
-    if (c.b == 0 && threadIdx.x == 0) {
+    if (c.b0 == 0 && threadIdx.x == 0) {
         const int it = st->iter + 1;
         st->error = error;
         st->rr = s[1];
@@ -1028,14 +1063,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_rescan(Grp g)
     const SysDev& S = *c.S;
     if (skip_action(S, 2)) return;
     const KVec<T> v = kv_of<T>(S);
-    double acc[2] = {0.0, 0.0};
-    vec_chunk(S.n, c.G, c.b, [&](int i) {
-        const T ri = v.r[i];
-        const T zi = ri / v.d[i];
-        acc[0] += (double)ri * (double)ri;
-        acc[1] += (double)zi * (double)ri;
-    });
-    store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, c.b);
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[2] = {0.0, 0.0};
+        vec_chunk(S.n, c.G, b, [&](int i) {
+            const T ri = v.r[i];
+            const T zi = ri / v.d[i];
+            acc[0] += (double)ri * (double)ri;
+            acc[1] += (double)zi * (double)ri;
+        });
+        store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, b);
+    }
 }
 
 // ... then the direction step the update interrupted: beta = (z,r)/rr_old,
@@ -1051,8 +1089,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_resume(Grp g)
     KrylovState* st = S.st;
     const T beta = (T)(s[1] / st->rr_old);
     const KVec<T> v = kv_of<T>(S);
-    vec_chunk(S.n, c.G, c.b, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
-    if (c.b == 0 && threadIdx.x == 0) {
+    vec_vblocks(c, S.n, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
+    if (c.b0 == 0 && threadIdx.x == 0) {
         st->rr = s[1];
         st->done = 0;
     }
@@ -1071,18 +1109,21 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(Grp g, int want)
     if (skip_action(S, want)) return;
     const MatView<VT> m = mv_of<VT>(S);
     const KVec<T> v = kv_of<T>(S);
-    double acc[3] = {0.0, 0.0, 0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.x, c.G, c.b, [&](int i, T ax) {
-        const T ri = v.b[i] - ax;
-        v.r0[i] = ri;
-        v.r[i] = ri;
-        v.v[i] = 0;
-        v.p[i] = 0;
-        acc[0] += (double)ri * (double)ri;
-        acc[1] += (double)ri * (double)ri;
-        acc[2] += (double)v.b[i] * (double)v.b[i];
-    });
-    store_partials<3>(acc, S.part, c.G, c.b);
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[3] = {0.0, 0.0, 0.0};
+        spmv_rows<FMT, VT, T, T>(m, v.x, c.G, b, [&](int i, T ax) {
+            const T ri = v.b[i] - ax;
+            v.r0[i] = ri;
+            v.r[i] = ri;
+            v.v[i] = 0;
+            v.p[i] = 0;
+            acc[0] += (double)ri * (double)ri;
+            acc[1] += (double)ri * (double)ri;
+            acc[2] += (double)v.b[i] * (double)v.b[i];
+        });
+        store_partials<3>(acc, S.part, c.G, b);
+    }
 }
 
 __global__ void k_bi_init_final(Grp g, int want)
@@ -1098,6 +1139,7 @@ __global__ void k_bi_init_final(Grp g, int want)
     st->iter = 0;
     st->max_iter = P.max_iter;
     st->done = 0;
+    st->pend = 0;
     st->rho0 = 1.0;
     st->alpha = 1.0;
     st->omega = 1.0;
@@ -1117,7 +1159,9 @@ __global__ void k_bi_init_final(Grp g, int want)
     if (st->error < P.tol) st->done = 1;
 }
 
-// B1: test of the previous iterate's r; rho1 = (r0,r); p update; p_h = p/d
+// B1: test of the previous iterate's r; rho1 = (r0,r); p update; p_h = p/d.
+// A convergence on s in B4 (pend) is turned into done here, one kernel
+// later, so that no block of B4 can see it before doing its own update.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
 {
@@ -1125,6 +1169,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
     const SysDev& S = *c.S;
     KrylovState* st = S.st;
     if (st->done) return;
+    if (st->pend) {
+        if (c.b0 == 0 && threadIdx.x == 0) st->done = 1;
+        return;
+    }
     double s[2];
     reduce_partials<2>(S.part, c.G, s);
     const double tol = S.prm->tol;
@@ -1132,14 +1180,14 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
     if (it > 0) {
         const double err = sqrt(s[0]) / st->bnorm;
         if (err < tol || fabs(st->omega) < DBL_MIN) {
-            if (c.b == 0 && threadIdx.x == 0) {
+            if (c.b0 == 0 && threadIdx.x == 0) {
                 st->error = err;
                 st->done = (err < tol) ? 1 : 3;
             }
             return;
         }
         if (it >= st->max_iter) {
-            if (c.b == 0 && threadIdx.x == 0) {
+            if (c.b0 == 0 && threadIdx.x == 0) {
                 st->error = err;
                 st->done = 2;
             }
@@ -1148,12 +1196,12 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
     }
     const double rho1 = s[1];
     if (fabs(rho1) < DBL_MIN) {
-        if (c.b == 0 && threadIdx.x == 0) st->done = 6;
+        if (c.b0 == 0 && threadIdx.x == 0) st->done = 6;
         return;
     }
     const KVec<T> v = kv_of<T>(S);
     if (it == 0) {
-        vec_chunk(S.n, c.G, c.b, [&](int i) {
+        vec_vblocks(c, S.n, [&](int i) {
             const T pi = v.r[i];
             v.p[i] = pi;
             v.ph[i] = pi / v.d[i];
@@ -1161,13 +1209,13 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
     } else {
         const T beta = (T)((rho1 / st->rho0) * (st->alpha / st->omega));
         const T omega = (T)st->omega;
-        vec_chunk(S.n, c.G, c.b, [&](int i) {
+        vec_vblocks(c, S.n, [&](int i) {
             const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
             v.p[i] = pi;
             v.ph[i] = pi / v.d[i];
         });
     }
-    if (c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
+    if (c.b0 == 0 && threadIdx.x == 0) st->rho1 = rho1;
 }
 
 // B2: v = A p_h; partial (r0,v); counts the iteration
@@ -1177,16 +1225,19 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv_v(Grp g)
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     KrylovState* st = S.st;
-    if (st->done) return;
+    if (st->done || st->pend) return;
     const MatView<VT> m = mv_of<VT>(S);
     const KVec<T> v = kv_of<T>(S);
-    double acc[1] = {0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.ph, c.G, c.b, [&](int i, T ax) {
-        v.v[i] = ax;
-        acc[0] += (double)v.r0[i] * (double)ax;
-    });
-    store_partials<1>(acc, S.part + 4 * (size_t)c.G, c.G, c.b);
-    if (c.b == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[1] = {0.0};
+        spmv_rows<FMT, VT, T, T>(m, v.ph, c.G, b, [&](int i, T ax) {
+            v.v[i] = ax;
+            acc[0] += (double)v.r0[i] * (double)ax;
+        });
+        store_partials<1>(acc, S.part + 4 * (size_t)c.G, c.G, b);
+    }
+    if (c.b0 == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
 // B3: alpha = rho1/(r0,v); s = r - alpha v; s_h = s/d; partial (s,s)
@@ -1196,57 +1247,63 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     KrylovState* st = S.st;
-    if (st->done) return;
+    if (st->done || st->pend) return;
     double sv[1];
     reduce_partials<1>(S.part + 4 * (size_t)c.G, c.G, sv);
     const double rho1 = st->rho1;
     const double alpha_d = rho1 / sv[0];
     const T alpha = (T)alpha_d;
     const KVec<T> v = kv_of<T>(S);
-    double acc[1] = {0.0};
-    vec_chunk(S.n, c.G, c.b, [&](int i) {
-        const T si = v.r[i] - alpha * v.v[i];
-        v.s[i] = si;
-        v.sh[i] = si / v.d[i];
-        acc[0] += (double)si * (double)si;
-    });
-    store_partials<1>(acc, S.part + 8 * (size_t)c.G, c.G, c.b);
-    if (c.b == 0 && threadIdx.x == 0) {
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[1] = {0.0};
+        vec_chunk(S.n, c.G, b, [&](int i) {
+            const T si = v.r[i] - alpha * v.v[i];
+            v.s[i] = si;
+            v.sh[i] = si / v.d[i];
+            acc[0] += (double)si * (double)si;
+        });
+        store_partials<1>(acc, S.part + 8 * (size_t)c.G, c.G, b);
+    }
+    if (c.b0 == 0 && threadIdx.x == 0) {
         st->alpha = alpha_d;
         st->rho0 = rho1;
     }
 }
 
-// B4: test on s (then x += alpha p_h and stop); else t = A s_h;
-// partials (t,t), (t,s)
+// B4: test on s (then x += alpha p_h and stop: pend, done in B1); else
+// t = A s_h; partials (t,t), (t,s)
 template <int FMT, typename VT, typename T>
 __global__ __launch_bounds__(kBlock) void k_bi_spmv_t(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     KrylovState* st = S.st;
-    if (st->done) return;
+    if (st->done || st->pend) return;
     double ss[1];
     reduce_partials<1>(S.part + 8 * (size_t)c.G, c.G, ss);
     const double err = sqrt(ss[0]) / st->bnorm;
     const KVec<T> v = kv_of<T>(S);
     if (err < S.prm->tol) {
         const T alpha = (T)st->alpha;
-        vec_chunk(S.n, c.G, c.b, [&](int i) { v.x[i] += alpha * v.ph[i]; });
-        if (c.b == 0 && threadIdx.x == 0) {
+        vec_vblocks(c, S.n, [&](int i) { v.x[i] += alpha * v.ph[i]; });
+        if (c.b0 == 0 && threadIdx.x == 0) {
             st->error = err;
-            st->done = 1;
+            st->pend = 1;
         }
         return;
     }
     const MatView<VT> m = mv_of<VT>(S);
-    double acc[2] = {0.0, 0.0};
-    spmv_rows<FMT, VT, T, T>(m, v.sh, c.G, c.b, [&](int i, T ax) {
-        v.t[i] = ax;
-        acc[0] += (double)ax * (double)ax;
-        acc[1] += (double)ax * (double)v.s[i];
-    });
-    store_partials<2>(acc, S.part + 12 * (size_t)c.G, c.G, c.b);
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[2] = {0.0, 0.0};
+        spmv_rows<FMT, VT, T, T>(m, v.sh, c.G, b, [&](int i, T ax) {
+            v.t[i] = ax;
+            acc[0] += (double)ax * (double)ax;
+            acc[1] += (double)ax * (double)v.s[i];
+        });
+        store_partials<2>(acc, S.part + 12 * (size_t)c.G, c.G, b);
+    }
 }
 
 // B5: omega = (t,s)/(t,t); x += alpha p_h + omega s_h; r = s - omega t;
@@ -1257,23 +1314,26 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     KrylovState* st = S.st;
-    if (st->done) return;
+    if (st->done || st->pend) return;
     double sv[2];
     reduce_partials<2>(S.part + 12 * (size_t)c.G, c.G, sv);
     const double omega_d = (sv[0] > DBL_MIN) ? sv[1] / sv[0] : 1.0;
     const T omega = (T)omega_d;
     const T alpha = (T)st->alpha;
     const KVec<T> v = kv_of<T>(S);
-    double acc[2] = {0.0, 0.0};
-    vec_chunk(S.n, c.G, c.b, [&](int i) {
-        v.x[i] += alpha * v.ph[i] + omega * v.sh[i];
-        const T ri = v.s[i] - omega * v.t[i];
-        v.r[i] = ri;
-        acc[0] += (double)ri * (double)ri;
-        acc[1] += (double)v.r0[i] * (double)ri;
-    });
-    store_partials<2>(acc, S.part, c.G, c.b);
-    if (c.b == 0 && threadIdx.x == 0) st->omega = omega_d;
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[2] = {0.0, 0.0};
+        vec_chunk(S.n, c.G, b, [&](int i) {
+            v.x[i] += alpha * v.ph[i] + omega * v.sh[i];
+            const T ri = v.s[i] - omega * v.t[i];
+            v.r[i] = ri;
+            acc[0] += (double)ri * (double)ri;
+            acc[1] += (double)v.r0[i] * (double)ri;
+        });
+        store_partials<2>(acc, S.part, c.G, b);
+    }
+    if (c.b0 == 0 && threadIdx.x == 0) st->omega = omega_d;
 }
 
 // ---- FP32-inside-FP64 iterative refinement helpers
@@ -1287,14 +1347,17 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
     const MatView<double> m = S.m64;
     const KVec<double> v = S.v64;
     double* r = S.r64;
-    double acc[2] = {0.0, 0.0};
-    spmv_rows<FMT, double, double, double>(m, v.x, c.G, c.b, [&](int i, double ax) {
-        const double ri = v.b[i] - ax;
-        r[i] = ri;
-        acc[0] += v.b[i] * v.b[i];
-        acc[1] += ri * ri;
-    });
-    store_partials<2>(acc, S.part, c.G, c.b);
+    for (int vb = 0; vb < c.nb; vb++) {
+        const int b = c.b0 + vb;
+        double acc[2] = {0.0, 0.0};
+        spmv_rows<FMT, double, double, double>(m, v.x, c.G, b, [&](int i, double ax) {
+            const double ri = v.b[i] - ax;
+            r[i] = ri;
+            acc[0] += v.b[i] * v.b[i];
+            acc[1] += ri * ri;
+        });
+        store_partials<2>(acc, S.part, c.G, b);
+    }
 }
 
 // x64 += unscale * x32: folds the FP32 correction of the recurrence that
@@ -1305,7 +1368,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_fold(Grp g)
     const SysDev& S = *c.S;
     if (!S.irs->started) return;
     const double u = S.prm->unscale;
-    vec_chunk(S.n, c.G, c.b, [&](int i) { S.v64.x[i] += u * (double)S.v32.x[i]; });
+    vec_vblocks(c, S.n, [&](int i) { S.v64.x[i] += u * (double)S.v32.x[i]; });
 }
 
 // One block per listed system: after k_ir_residual, the refinement step.
@@ -1347,6 +1410,7 @@ __global__ void k_ir_decide(Grp g)
             return;
         }
     } else {
+        // st.done == 0 with pend set: BiCGStab converged on s
         inner_iters = st.iter;
         inner_status = st.done == 2 ? 2 : (st.done == 3 || st.done == 6) ? 3 : 1;
         R.total = P.ir_reliable ? st.iter : R.total + max(st.iter, 1);
@@ -1398,7 +1462,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g)
     if (act == 0) return;
     const double scale = S.prm->scale;
     float* dst = act == 1 ? S.v32.b : S.v32.r;
-    vec_chunk(S.n, c.G, c.b, [&](int i) {
+    vec_vblocks(c, S.n, [&](int i) {
         dst[i] = (float)(S.r64[i] * scale);
         S.v32.x[i] = 0.0f;
     });

@@ -91,7 +91,7 @@ struct KrylovState {
                       // (BiCGStab: omega), 4 skipped by CheckNormRHS,
                       // 5 reliable-update request, 6 BiCGStab rho breakdown
     int max_iter;
-    int pad;
+    int pend;         // BiCGStab: converged on s, x updated; done follows
 };
 
 }  // namespace ogsx
