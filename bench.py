@@ -59,6 +59,11 @@ def gpu_leg(sc, precision, steps, warmup):
     ms, launches, bpl = s.profile_read(kind)
     ms64, l64, b64 = s.profile_read(ox._lib.PROF_SPMV64)
     s.profile(False)
+    # the dominant kernel alone: the solve's SpMV kernel (k_cg_spmv, tuned
+    # format) on the last step's matrix, 50 launches timed with HIP events on
+    # the system's stream.  The in-solve average above also counts launches
+    # that found the recurrence already stopped (early exit within a batch).
+    k_ms, k_bytes = s.bench_spmv(precision == ox.FP32_IR, 50)
     solves = len(sc.steps) * steps
     last = infos[-1]
     assert all(i["converged"] == 1 for job in infos for i in job), infos[-1]
@@ -68,11 +73,15 @@ def gpu_leg(sc, precision, steps, warmup):
         mdof_solves_per_s=solves * sc.n / dt / 1e6,
         iterations=[i["iterations"] for i in last], outer=[i["outer"] for i in last],
         errors=[i["error"] for i in last],
-        spmv=dict(avg_ms=ms / max(launches, 1), launches=launches, bytes_per_launch=bpl,
-                  gbs=bpl / (ms / max(launches, 1)) / 1e6 if launches else None),
+        spmv=dict(avg_ms=k_ms, bytes_per_launch=k_bytes, gbs=k_bytes / k_ms / 1e6, launches=50,
+                  kernel_alone=True),
+        spmv_in_solve=dict(avg_ms=ms / max(launches, 1), launches=launches, bytes_per_launch=bpl,
+                           gbs=bpl / (ms / max(launches, 1)) / 1e6 if launches else None,
+                           iterations=sum(i["iterations"] for job in infos for i in job)),
         spmv64=dict(avg_ms=ms64 / max(l64, 1), launches=l64, bytes_per_launch=b64,
                     gbs=b64 / (ms64 / max(l64, 1)) / 1e6 if l64 else None),
         spmv_variant=s.spmv_variant(),
+        spmv_format=s.spmv_format(precision == ox.FP32_IR),
         spmv_tuning=s.spmv_tuning(),
     )
 
@@ -287,7 +296,8 @@ def main():
     roof = dict(bound="hbm", achieved=sp["gbs"], peak=peak, unit="GB/s",
                 frac=(sp["gbs"] / peak) if sp["gbs"] else None, traffic=None,
                 kernel=("k_cg_spmv<float>" if best is ir else "k_cg_spmv<double>"),
-                avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"])
+                avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"],
+                format=best["spmv_format"])
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         tr = json.load(open(tf)).get(roof["kernel"])

@@ -311,6 +311,14 @@ class System:
         check(lib().ogsx_get_spmv_variant(self.h, C.byref(v)), "ogsx_get_spmv_variant")
         return v.value
 
+    def spmv_format(self, fp32=False) -> int:
+        """Tuned SpMV format of the FP64 (or, for IR systems, FP32) matrix:
+        the fastest entry of that half of the autotuner's table."""
+        t = self.spmv_tuning()
+        half = len(t) // 2 if self.precision == FP32_IR else len(t)
+        part = t[half:] if fp32 else t[:half]
+        return min(part, key=lambda e: e[1])[0]
+
     def bench_spmv(self, fp32=False, reps=20):
         ms, b = C.c_double(), C.c_double()
         check(lib().ogsx_bench_spmv(self.h, int(bool(fp32)), int(reps), C.byref(ms), C.byref(b)),

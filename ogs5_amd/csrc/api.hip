@@ -106,6 +106,9 @@ struct SolveCtx {
     int total = 0, updates = 0;
     int converged = 0, ret = 0;
     double error = 0.0, bnorm = 0.0;
+    // convergence rate of the running recurrence segment, for batch sizing
+    int seg_key = -1, seg_it0 = 0;
+    double seg_e0 = 0.0;
 };
 }  // namespace
 
@@ -291,7 +294,7 @@ void build_sell(const PatternHost& h, std::vector<int>& sptr, std::vector<int>& 
 // Brings the SELL value copies up to date with A (after assembly / BC).
 int sync_sell(ogsx_system* s, bool want64, bool want32)
 {
-    const Grp g1{s->dev, nullptr, nullptr, 1};
+    const Grp g1{s->dev, nullptr, nullptr};
     if (want64 && s->sval64_version != s->a_version) {
         k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1);
         s->sval64_version = s->a_version;
@@ -447,28 +450,39 @@ int fmt_of(const ogsx_system* s)
     return sizeof(VT) == 8 ? s->fmt64 : s->fmt32;
 }
 
-// Autotuner: times every SpMV format (tiled with 16/32-bit columns, vector
-// CRS with 8..64 lanes per row) on this matrix and keeps the fastest.
+// The solve's SpMV kernel (k_cg_spmv: s = A p and the partial (p, s)) on
+// this system alone; the caller has set the state to running.
 template <typename VT, typename T>
-int tune_one(ogsx_system* s, const VT* val, const T* x, T* y, int* fmt_out)
+void launch_solve_spmv(ogsx_system* s, int fmt)
 {
-    // Only formats that sum every row in CRS order are candidates, so the
-    // choice (which depends on timing) never changes the results.
+    FMT_DISPATCH(fmt, (k_cg_spmv<L, VT, T><<<s->G, kBlock, 0, s->stream>>>(
+                          Grp{s->dev, nullptr, nullptr})));
+}
+
+// Autotuner: times the solve's SpMV kernel in every format that sums each
+// row in CRS order (tiled with 16/32-bit columns, SELL-64 variants) on this
+// matrix and keeps the fastest.  All candidates give bit-identical results,
+// so the choice (which depends on timing) never changes a result.  The
+// Krylov vectors are scratch at this point; p is set to 1.
+template <typename VT, typename T>
+int tune_one(ogsx_system* s, int* fmt_out)
+{
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
                          kFmtSell16X2,  kFmtSell32,   kFmtTile16,   kFmtTile32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const MatView<VT> mv = mat_view(s, val);
+    T* p = sizeof(T) == 8 ? (T*)s->v64.p : (T*)s->v32.p;
+    k_fill<T><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, p, (T)1);
+    k_set_done<<<1, 64, 0, s->stream>>>(s->st, 0);
     int best = kFmtTile32;
     float best_ms = 1e30f;
     const int reps = 4;
     for (int f : cands) {
         if (!valid_fmt(s, f)) continue;
-        FMT_DISPATCH(f, (k_spmv<L, VT, T><<<s->G, kBlock, 0, s->stream>>>(mv, x, y)));
+        launch_solve_spmv<VT, T>(s, f);
         (void)hipEventRecord(e0, s->stream);
-        for (int r = 0; r < reps; r++)
-            FMT_DISPATCH(f, (k_spmv<L, VT, T><<<s->G, kBlock, 0, s->stream>>>(mv, x, y)));
+        for (int r = 0; r < reps; r++) launch_solve_spmv<VT, T>(s, f);
         (void)hipEventRecord(e1, s->stream);
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -479,6 +493,7 @@ int tune_one(ogsx_system* s, const VT* val, const T* x, T* y, int* fmt_out)
             best = f;
         }
     }
+    k_set_done<<<1, 64, 0, s->stream>>>(s->st, 4);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIP_TRY(hipGetLastError());
@@ -490,10 +505,9 @@ int autotune(ogsx_system* s)
 {
     if (s->fmt64 == 0) {
         s->tune_log.clear();
-        OK_TRY(tune_one<double, double>(s, s->A, s->v64.x, s->v64.s, &s->fmt64));
+        OK_TRY((tune_one<double, double>(s, &s->fmt64)));
     }
-    if (s->precision == OGSX_FP32_IR && s->fmt32 == 0)
-        OK_TRY(tune_one<float, float>(s, s->A32, s->v32.x, s->v32.s, &s->fmt32));
+    if (s->precision == OGSX_FP32_IR && s->fmt32 == 0) OK_TRY((tune_one<float, float>(s, &s->fmt32)));
     return OGSX_OK;
 }
 
@@ -1334,25 +1348,25 @@ int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* 
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    // the solve's own SpMV kernel (k_cg_spmv), on p = 1
     if (!fp32) {
         OK_TRY(sync_sell(s, is_sell(s->fmt64), false));
-        const MatView<double> mv = mat_view(s, (const double*)s->A);
+        k_fill<double><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v64.p, 1.0);
+        k_set_done<<<1, 64, 0, s->stream>>>(s->st, 0);
         (void)hipEventRecord(e0, s->stream);
-        for (int r = 0; r < reps; r++)
-            FMT_DISPATCH(s->fmt64, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
-                                       mv, s->v64.x, s->v64.s)));
+        for (int r = 0; r < reps; r++) launch_solve_spmv<double, double>(s, s->fmt64);
         (void)hipEventRecord(e1, s->stream);
     } else {
         if (!is_sell(s->fmt32))
-            k_cast_f32_g<<<s->G, kBlock, 0, s->stream>>>(Grp{s->dev, nullptr, nullptr, 1});
+            k_cast_f32_g<<<s->G, kBlock, 0, s->stream>>>(Grp{s->dev, nullptr, nullptr});
         OK_TRY(sync_sell(s, false, is_sell(s->fmt32)));
-        const MatView<float> mv = mat_view(s, (const float*)s->A32);
+        k_fill<float><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v32.p, 1.0f);
+        k_set_done<<<1, 64, 0, s->stream>>>(s->st, 0);
         (void)hipEventRecord(e0, s->stream);
-        for (int r = 0; r < reps; r++)
-            FMT_DISPATCH(s->fmt32, (k_spmv<L, float, float><<<s->G, kBlock, 0, s->stream>>>(
-                                       mv, s->v32.x, s->v32.s)));
+        for (int r = 0; r < reps; r++) launch_solve_spmv<float, float>(s, s->fmt32);
         (void)hipEventRecord(e1, s->stream);
     }
+    k_set_done<<<1, 64, 0, s->stream>>>(s->st, 4);
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));

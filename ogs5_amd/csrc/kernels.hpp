@@ -23,6 +23,18 @@
 
 namespace ogsx {
 
+// Device pointers read from descriptors in memory (not kernel arguments) are
+// generic to the compiler, which then emits flat loads/stores.  Kernels
+// re-type them as global-address-space pointers (gl) so they get
+// global_load/global_store; the hot loops measured ~40% slower with flat.
+#define OGSX_GL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ OGSX_GL T* gl(T* p)
+{
+    return (OGSX_GL T*)p;
+}
+
 // ------------------------------------------------------------------------
 // generic helpers
 // ------------------------------------------------------------------------
@@ -45,7 +57,8 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // Block-reduces NV values and stores them as partials[k*G + b].
 template <int NV>
-__device__ __forceinline__ void store_partials(double (&v)[NV], double* partials, int G, int b)
+__device__ __forceinline__ void store_partials(double (&v)[NV], OGSX_GL double* partials, int G,
+                                               int b)
 {
     __shared__ double sm[NV][kBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -64,13 +77,12 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* partials
             partials[(size_t)k * G + b] = s;
         }
     }
-    __syncthreads();  // sm is reused by the block's next virtual block
 }
 
 // Sums G partials per value in a fixed order; every thread gets the result.
 // G is the grid size of the producer, which equals this kernel's grid.
 template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* partials, int G,
+__device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, int G,
                                                 double (&out)[NV])
 {
     __shared__ double sm[NV][kBlock / 64];
@@ -100,9 +112,10 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int G,
 // the row sum.  Lanes of one group share `row`, so they enter and leave the
 // row loop together and the width-LPR shuffles stay inside active lanes.
 template <int LPR, typename VT, typename XT, typename ACC>
-__device__ __forceinline__ ACC row_dot(const int* __restrict__ ptr, const int* __restrict__ col,
-                                       const VT* __restrict__ val, const XT* __restrict__ x,
-                                       int row, int lane)
+__device__ __forceinline__ ACC row_dot(const OGSX_GL int* __restrict__ ptr,
+                                       const OGSX_GL int* __restrict__ col,
+                                       const OGSX_GL VT* __restrict__ val,
+                                       const OGSX_GL XT* __restrict__ x, int row, int lane)
 {
     const int beg = ptr[row], end = ptr[row + 1];
     ACC s = 0;
@@ -115,9 +128,10 @@ __device__ __forceinline__ ACC row_dot(const int* __restrict__ ptr, const int* _
 // Loops the block's chunk of rows with LPR lanes per row and calls
 // epi(row, Ax_row) on the group's lane 0.
 template <int LPR, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_chunk(int n, const int* __restrict__ ptr,
-                                           const int* __restrict__ col, const VT* __restrict__ val,
-                                           const XT* __restrict__ x, int G, int b, Epi&& epi)
+__device__ __forceinline__ void spmv_chunk(int n, const OGSX_GL int* __restrict__ ptr,
+                                           const OGSX_GL int* __restrict__ col,
+                                           const OGSX_GL VT* __restrict__ val,
+                                           const OGSX_GL XT* __restrict__ x, int G, int b, Epi&& epi)
 {
     constexpr int RPB = kBlock / LPR;
     int beg, end;
@@ -163,6 +177,35 @@ struct MatView {
     const VT* sval;
 };
 
+// MatView with global-address-space pointers, as the device loops use it.
+template <typename VT>
+struct GMat {
+    int n;
+    int n_tiles;
+    const OGSX_GL int* ptr;
+    const OGSX_GL int* col;
+    const OGSX_GL uint16_t* col16;
+    const OGSX_GL VT* val;
+    const OGSX_GL int* tile_row;
+    const OGSX_GL int* tile_cb;
+    int n_slices;
+    const OGSX_GL int* sell_ptr;
+    const OGSX_GL int* sell_w;
+    const OGSX_GL int* sell_cb;
+    const OGSX_GL uint16_t* sell_c16;
+    const OGSX_GL int* sell_c32;
+    const OGSX_GL VT* sval;
+};
+
+template <typename VT>
+__device__ __forceinline__ GMat<VT> gmat(const MatView<VT>& m)
+{
+    return GMat<VT>{m.n,          m.n_tiles,       gl(m.ptr),      gl(m.col),
+                    gl(m.col16),  gl(m.val),       gl(m.tile_row), gl(m.tile_cb),
+                    m.n_slices,   gl(m.sell_ptr),  gl(m.sell_w),   gl(m.sell_cb),
+                    gl(m.sell_c16), gl(m.sell_c32), gl(m.sval)};
+}
+
 constexpr int kTileU = 8;                 // nonzeros per thread per tile
 constexpr int kTileNnz = kTileU * kBlock; // nonzeros per tile (2048)
 constexpr int kFmtTile32 = 1;             // tiled, 32-bit columns
@@ -185,7 +228,7 @@ constexpr int kSellC = 64;                // rows per slice = wavefront size
 // are summed left to right over rounded products, as the SELL kernel does.
 // A tile whose single row exceeds kTileNnz is summed by the whole block.
 template <int C16, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __restrict__ x,
+__device__ __forceinline__ void spmv_tiles(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
                                            int G, int b, Epi&& epi)
 {
 #pragma clang fp contract(off)
@@ -259,15 +302,15 @@ __device__ __forceinline__ void spmv_tiles(const MatView<VT>& m, const XT* __res
 // instruction), rows need no cross-lane reduction, and for meshes numbered
 // along a grid line the 64 lanes' x gathers of one entry j touch 64
 // consecutive x values.  Blocks take contiguous slice ranges, XCD-aware.
-template <typename T>
-__device__ __forceinline__ T ld(const T* p, bool nt)
+template <typename P>
+__device__ __forceinline__ auto ld(P p, bool nt)
 {
     return nt ? __builtin_nontemporal_load(p) : *p;
 }
 
 template <int C16, int U, bool NT, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_sell(const MatView<VT>& m, const XT* __restrict__ x, int G,
-                                          int b, Epi&& epi)
+__device__ __forceinline__ void spmv_sell(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                          int G, int b, Epi&& epi)
 {
 #pragma clang fp contract(off)
     int sb, se;
@@ -311,7 +354,7 @@ __device__ __forceinline__ void spmv_sell(const MatView<VT>& m, const XT* __rest
 // Two slices per wavefront in one loop: twice the independent loads in
 // flight; each row is still summed in CRS order.
 template <int U, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_sell_x2(const MatView<VT>& m, const XT* __restrict__ x,
+__device__ __forceinline__ void spmv_sell_x2(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
                                              int G, int b, Epi&& epi)
 {
 #pragma clang fp contract(off)
@@ -372,8 +415,8 @@ __device__ __forceinline__ void spmv_sell_x2(const MatView<VT>& m, const XT* __r
 // Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled or
 // SELL-64 format.
 template <int FMT, typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_rows(const MatView<VT>& m, const XT* __restrict__ x, int G,
-                                          int b, Epi&& epi)
+__device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                          int G, int b, Epi&& epi)
 {
     if constexpr (FMT == kFmtTile32)
         spmv_tiles<0, VT, XT, ACC>(m, x, G, b, epi);
@@ -664,8 +707,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(MatView<VT> m, const XT* __rest
                                                  XT* __restrict__ y)
 {
     using ACC = XT;
-    spmv_rows<FMT, VT, XT, ACC>(m, x, gridDim.x, blockIdx.x,
-                                [&](int r, ACC s) { y[r] = (XT)s; });
+    OGSX_GL XT* __restrict__ yg = gl(y);
+    spmv_rows<FMT, VT, XT, ACC>(gmat(m), gl(x), gridDim.x, blockIdx.x,
+                                [&](int r, ACC s) { yg[r] = (XT)s; });
 }
 
 // ------------------------------------------------------------------------
@@ -689,6 +733,30 @@ struct KVec {
     T* ph;
     T* sh;
 };
+
+// KVec with global-address-space pointers.
+template <typename T>
+struct GVec {
+    OGSX_GL T* x;
+    OGSX_GL T* b;
+    OGSX_GL T* d;
+    OGSX_GL T* r;
+    OGSX_GL T* p;
+    OGSX_GL T* s;
+    OGSX_GL T* z;
+    OGSX_GL T* r0;
+    OGSX_GL T* v;
+    OGSX_GL T* t;
+    OGSX_GL T* ph;
+    OGSX_GL T* sh;
+};
+
+template <typename T>
+__device__ __forceinline__ GVec<T> gvec(const KVec<T>& k)
+{
+    return GVec<T>{gl(k.x), gl(k.b), gl(k.d),  gl(k.r), gl(k.p),  gl(k.s),
+                   gl(k.z), gl(k.r0), gl(k.v), gl(k.t), gl(k.ph), gl(k.sh)};
+}
 
 // Per-solve parameters of one system.  The recurrence fields are written by
 // the host (FP64 solves) or by k_ir_decide (the FP32 recurrence of IR); the
@@ -757,40 +825,29 @@ __device__ __forceinline__ const KVec<double>& kv_of<double>(const SysDev& S) { 
 template <>
 __device__ __forceinline__ const KVec<float>& kv_of<float>(const SysDev& S) { return S.v32; }
 
-// Launch geometry.  A system's rows are split into G chunks ("virtual
-// blocks", each with its own partial sums); a launched block works on vpb
-// consecutive virtual blocks of one system, one after the other, with the
-// same thread -> row mapping and reduction order as if each were a block of
-// its own.  So vpb changes the number of launched blocks, never a result.
-// blk2sys = nullptr: one system, grid = ceil(G / vpb).  Otherwise blk2sys
-// maps a launched block to its system and blk0[i] is system i's first
-// launched block.  One-block-per-system kernels use blk2sys as the list.
+// Launch geometry: blk2sys = nullptr for one system (grid = its G blocks);
+// otherwise blk2sys maps a launched block to its system and blk0[i] is
+// system i's first block.  Kernels with one block per system use blk2sys as
+// the block -> system list.  (Launching fewer blocks that each work through
+// several of a system's G row chunks was measured on a 256-system ensemble:
+// 4-7% slower, and the loop costs ~10 VGPRs in the SpMV kernels.)
 struct Grp {
     const SysDev* sys;
     const int* blk2sys;
     const int* blk0;
-    int vpb;
 };
 
 struct Ctx {
     const SysDev* S;
-    int G, b0, nb;  // virtual blocks [b0, b0 + nb) of the system's G
+    int G, b;  // the block is chunk b of the system's G
 };
 
 __device__ __forceinline__ Ctx ctx_of(const Grp& g)
 {
-    const SysDev* S;
-    int first;
-    if (!g.blk2sys) {
-        S = g.sys;
-        first = (int)blockIdx.x * g.vpb;
-    } else {
-        const int si = g.blk2sys[blockIdx.x];
-        S = g.sys + si;
-        first = ((int)blockIdx.x - g.blk0[si]) * g.vpb;
-    }
-    const int G = S->G;
-    return Ctx{S, G, first, min(g.vpb, G - first)};
+    if (!g.blk2sys) return Ctx{g.sys, g.sys->G, (int)blockIdx.x};
+    const int si = g.blk2sys[blockIdx.x];
+    const SysDev* S = g.sys + si;
+    return Ctx{S, S->G, (int)blockIdx.x - g.blk0[si]};
 }
 
 __device__ __forceinline__ const SysDev* sys_of_block(const Grp& g)
@@ -798,11 +855,11 @@ __device__ __forceinline__ const SysDev* sys_of_block(const Grp& g)
     return g.sys + (g.blk2sys ? g.blk2sys[blockIdx.x] : (int)blockIdx.x);
 }
 
-// Vector body over the rows of the block's virtual blocks.
+// Vector body over the rows of the block's chunk.
 template <typename Body>
 __device__ __forceinline__ void vec_vblocks(const Ctx& c, int n, Body&& body)
 {
-    for (int v = 0; v < c.nb; v++) vec_chunk(n, c.G, c.b0 + v, body);
+    vec_chunk(n, c.G, c.b, body);
 }
 
 // Jacobi diagonal: Precond_Jacobi's a_ii with |a_ii| < DBL_MIN -> 1
@@ -812,10 +869,12 @@ __global__ __launch_bounds__(kBlock) void k_diag(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    T* d = kv_of<T>(S).d;
-    const int jac = S.prm->jacobi;
+    OGSX_GL T* d = gl(kv_of<T>(S).d);
+    const OGSX_GL int* dp = gl(S.diagpos);
+    const OGSX_GL double* A = gl(S.A);
+    const int jac = gl(S.prm)->jacobi;
     vec_vblocks(c, S.n, [&](int i) {
-        double v = (jac && S.diagpos[i] >= 0) ? S.A[S.diagpos[i]] : 1.0;
+        double v = (jac && dp[i] >= 0) ? A[dp[i]] : 1.0;
         if (fabs(v) < DBL_MIN) v = 1.0;
         d[i] = (T)v;
     });
@@ -827,14 +886,16 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    VT* sv;
+    OGSX_GL VT* sv;
     if constexpr (sizeof(VT) == 8)
-        sv = S.sval64;
+        sv = gl(S.sval64);
     else
-        sv = S.sval32;
+        sv = gl(S.sval32);
+    const OGSX_GL int* src = gl(S.sell_src);
+    const OGSX_GL double* A = gl(S.A);
     vec_vblocks(c, S.npad, [&](int p) {
-        const int k = S.sell_src[p];
-        sv[p] = k >= 0 ? (VT)S.A[k] : (VT)0;
+        const int k = src[p];
+        sv[p] = k >= 0 ? (VT)A[k] : (VT)0;
     });
 }
 
@@ -842,14 +903,16 @@ __global__ __launch_bounds__(kBlock) void k_cast_f32_g(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    vec_vblocks(c, S.nnz, [&](int k) { S.A32[k] = (float)S.A[k]; });
+    OGSX_GL float* A32 = gl(S.A32);
+    const OGSX_GL double* A = gl(S.A);
+    vec_vblocks(c, S.nnz, [&](int k) { A32[k] = (float)A[k]; });
 }
 
 // IR event kernels run only for the systems whose last k_ir_decide chose
 // `action` (1 new recurrence, 2 resume); FP64 solves pass want = 0.
 __device__ __forceinline__ bool skip_action(const SysDev& S, int want)
 {
-    return want != 0 && S.irs->action != want;
+    return want != 0 && gl(S.irs)->action != want;
 }
 
 // ---- Jacobi-preconditioned CG (Linear_EQS::CG, FEM/equation_class.cpp:1129)
@@ -861,10 +924,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Grp g, int want)
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     if (skip_action(S, want)) return;
-    const MatView<VT> m = mv_of<VT>(S);
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[3] = {0.0, 0.0, 0.0};
         spmv_rows<FMT, VT, T, T>(m, v.x, c.G, b, [&](int i, T ax) {
             const T ri = v.b[i] - ax;
@@ -875,7 +938,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Grp g, int want)
             acc[1] += (double)ri * (double)ri;
             acc[2] += (double)ri * (double)zi;
         });
-        store_partials<3>(acc, S.part, c.G, b);
+        store_partials<3>(acc, gl(S.part), c.G, b);
     }
 }
 
@@ -885,10 +948,10 @@ __global__ void k_cg_init_final(Grp g, int want)
     const SysDev& S = *sys_of_block(g);
     if (skip_action(S, want)) return;
     double s[3];
-    reduce_partials<3>(S.part, S.G, s);
+    reduce_partials<3>(gl(S.part), S.G, s);
     if (threadIdx.x != 0) return;
-    KrylovState* st = S.st;
-    const KrylovParams P = *S.prm;
+    OGSX_GL KrylovState* st = gl(S.st);
+    const KrylovParams P = *S.prm;  // one block per system: a generic load is fine
     const double bnorm_new = sqrt(s[0]);
     st->iter = 0;
     st->max_iter = P.max_iter;
@@ -921,17 +984,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    if (S.st->done) return;
-    const MatView<VT> m = mv_of<VT>(S);
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    if (gl(S.st)->done) return;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[1] = {0.0};
         spmv_rows<FMT, VT, T, T>(m, v.p, c.G, b, [&](int i, T ax) {
             v.s[i] = ax;
             acc[0] += (double)v.p[i] * (double)ax;
         });
-        store_partials<1>(acc, S.part, c.G, b);
+        store_partials<1>(acc, gl(S.part), c.G, b);
     }
 }
 
@@ -941,20 +1004,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
     double ps[1];
-    reduce_partials<1>(S.part, c.G, ps);
+    reduce_partials<1>(gl(S.part), c.G, ps);
     const double rr = st->rr;
     const T alpha = (T)(rr / ps[0]);
-    const KVec<T> v = kv_of<T>(S);
-    T* __restrict__ x = v.x;
-    T* __restrict__ r = v.r;
-    const T* __restrict__ p = v.p;
-    const T* __restrict__ sv = v.s;
-    const T* __restrict__ d = v.d;
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    OGSX_GL T* __restrict__ x = v.x;
+    OGSX_GL T* __restrict__ r = v.r;
+    const OGSX_GL T* __restrict__ p = v.p;
+    const OGSX_GL T* __restrict__ sv = v.s;
+    const OGSX_GL T* __restrict__ d = v.d;
+    {
+        const int b = c.b;
         double acc[2] = {0.0, 0.0};
         int beg, end;
         chunk_of(S.n, c.G, b, beg, end);
@@ -990,9 +1053,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
             acc[0] += (double)rn * (double)rn;
             acc[1] += (double)zi * (double)rn;
         }
-        store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, b);
+        store_partials<2>(acc, gl(S.part) + 4 * (size_t)c.G, c.G, b);
     }
-    if (c.b0 == 0 && threadIdx.x == 0) {
+    if (c.b == 0 && threadIdx.x == 0) {
         st->rr_old = rr;
         st->alpha = rr / ps[0];
     }
@@ -1007,15 +1070,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
     double s[2];
-    reduce_partials<2>(S.part + 4 * (size_t)c.G, c.G, s);
+    reduce_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, s);
     const double rnorm = sqrt(s[0]);
     const double error = rnorm / st->bnorm;
-    const double tol = S.prm->tol;
-    if (error < tol || rnorm < S.prm->upd_thresh) {
-        if (c.b0 == 0 && threadIdx.x == 0) {
+    const double tol = gl(S.prm)->tol;
+    if (error < tol || rnorm < gl(S.prm)->upd_thresh) {
+        if (c.b == 0 && threadIdx.x == 0) {
             st->error = error;
             st->iter = st->iter + 1;
             st->done = error < tol ? 1 : 5;
@@ -1023,13 +1086,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
         return;
     }
     const T beta = (T)(s[1] / st->rr_old);
-    const KVec<T> v = kv_of<T>(S);
-    T* __restrict__ p = v.p;
-    const T* __restrict__ r = v.r;
-    const T* __restrict__ d = v.d;
-    for (int vb = 0; vb < c.nb; vb++) {
+    const GVec<T> v = gvec(kv_of<T>(S));
+    OGSX_GL T* __restrict__ p = v.p;
+    const OGSX_GL T* __restrict__ r = v.r;
+    const OGSX_GL T* __restrict__ d = v.d;
+    {
         int beg, end;
-        chunk_of(S.n, c.G, c.b0 + vb, beg, end);
+        chunk_of(S.n, c.G, c.b, beg, end);
         constexpr int U = 4;
         int i = beg + threadIdx.x;
         for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
@@ -1046,7 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
         }
         for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
     }
-    if (c.b0 == 0 && threadIdx.x == 0) {
+    if (c.b == 0 && threadIdx.x == 0) {
         const int it = st->iter + 1;
         st->error = error;
         st->rr = s[1];
@@ -1062,9 +1125,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_rescan(Grp g)
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     if (skip_action(S, 2)) return;
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[2] = {0.0, 0.0};
         vec_chunk(S.n, c.G, b, [&](int i) {
             const T ri = v.r[i];
@@ -1072,7 +1135,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_rescan(Grp g)
             acc[0] += (double)ri * (double)ri;
             acc[1] += (double)zi * (double)ri;
         });
-        store_partials<2>(acc, S.part + 4 * (size_t)c.G, c.G, b);
+        store_partials<2>(acc, gl(S.part) + 4 * (size_t)c.G, c.G, b);
     }
 }
 
@@ -1085,12 +1148,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_resume(Grp g)
     const SysDev& S = *c.S;
     if (skip_action(S, 2)) return;
     double s[2];
-    reduce_partials<2>(S.part + 4 * (size_t)c.G, c.G, s);
-    KrylovState* st = S.st;
+    reduce_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, s);
+    OGSX_GL KrylovState* st = gl(S.st);
     const T beta = (T)(s[1] / st->rr_old);
-    const KVec<T> v = kv_of<T>(S);
+    const GVec<T> v = gvec(kv_of<T>(S));
     vec_vblocks(c, S.n, [&](int i) { v.p[i] = v.r[i] / v.d[i] + beta * v.p[i]; });
-    if (c.b0 == 0 && threadIdx.x == 0) {
+    if (c.b == 0 && threadIdx.x == 0) {
         st->rr = s[1];
         st->done = 0;
     }
@@ -1107,10 +1170,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(Grp g, int want)
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     if (skip_action(S, want)) return;
-    const MatView<VT> m = mv_of<VT>(S);
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[3] = {0.0, 0.0, 0.0};
         spmv_rows<FMT, VT, T, T>(m, v.x, c.G, b, [&](int i, T ax) {
             const T ri = v.b[i] - ax;
@@ -1122,7 +1185,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(Grp g, int want)
             acc[1] += (double)ri * (double)ri;
             acc[2] += (double)v.b[i] * (double)v.b[i];
         });
-        store_partials<3>(acc, S.part, c.G, b);
+        store_partials<3>(acc, gl(S.part), c.G, b);
     }
 }
 
@@ -1131,10 +1194,10 @@ __global__ void k_bi_init_final(Grp g, int want)
     const SysDev& S = *sys_of_block(g);
     if (skip_action(S, want)) return;
     double s[3];
-    reduce_partials<3>(S.part, S.G, s);
+    reduce_partials<3>(gl(S.part), S.G, s);
     if (threadIdx.x != 0) return;
-    KrylovState* st = S.st;
-    const KrylovParams P = *S.prm;
+    OGSX_GL KrylovState* st = gl(S.st);
+    const KrylovParams P = *S.prm;  // one block per system: a generic load is fine
     const double bnorm_new = sqrt(s[2]);
     st->iter = 0;
     st->max_iter = P.max_iter;
@@ -1167,27 +1230,27 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
     if (st->pend) {
-        if (c.b0 == 0 && threadIdx.x == 0) st->done = 1;
+        if (c.b == 0 && threadIdx.x == 0) st->done = 1;
         return;
     }
     double s[2];
-    reduce_partials<2>(S.part, c.G, s);
-    const double tol = S.prm->tol;
+    reduce_partials<2>(gl(S.part), c.G, s);
+    const double tol = gl(S.prm)->tol;
     const int it = st->iter;
     if (it > 0) {
         const double err = sqrt(s[0]) / st->bnorm;
         if (err < tol || fabs(st->omega) < DBL_MIN) {
-            if (c.b0 == 0 && threadIdx.x == 0) {
+            if (c.b == 0 && threadIdx.x == 0) {
                 st->error = err;
                 st->done = (err < tol) ? 1 : 3;
             }
             return;
         }
         if (it >= st->max_iter) {
-            if (c.b0 == 0 && threadIdx.x == 0) {
+            if (c.b == 0 && threadIdx.x == 0) {
                 st->error = err;
                 st->done = 2;
             }
@@ -1196,10 +1259,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
     }
     const double rho1 = s[1];
     if (fabs(rho1) < DBL_MIN) {
-        if (c.b0 == 0 && threadIdx.x == 0) st->done = 6;
+        if (c.b == 0 && threadIdx.x == 0) st->done = 6;
         return;
     }
-    const KVec<T> v = kv_of<T>(S);
+    const GVec<T> v = gvec(kv_of<T>(S));
     if (it == 0) {
         vec_vblocks(c, S.n, [&](int i) {
             const T pi = v.r[i];
@@ -1215,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
             v.ph[i] = pi / v.d[i];
         });
     }
-    if (c.b0 == 0 && threadIdx.x == 0) st->rho1 = rho1;
+    if (c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
 }
 
 // B2: v = A p_h; partial (r0,v); counts the iteration
@@ -1224,20 +1287,20 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv_v(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
-    const MatView<VT> m = mv_of<VT>(S);
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[1] = {0.0};
         spmv_rows<FMT, VT, T, T>(m, v.ph, c.G, b, [&](int i, T ax) {
             v.v[i] = ax;
             acc[0] += (double)v.r0[i] * (double)ax;
         });
-        store_partials<1>(acc, S.part + 4 * (size_t)c.G, c.G, b);
+        store_partials<1>(acc, gl(S.part) + 4 * (size_t)c.G, c.G, b);
     }
-    if (c.b0 == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+    if (c.b == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
 // B3: alpha = rho1/(r0,v); s = r - alpha v; s_h = s/d; partial (s,s)
@@ -1246,16 +1309,16 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
     double sv[1];
-    reduce_partials<1>(S.part + 4 * (size_t)c.G, c.G, sv);
+    reduce_partials<1>(gl(S.part) + 4 * (size_t)c.G, c.G, sv);
     const double rho1 = st->rho1;
     const double alpha_d = rho1 / sv[0];
     const T alpha = (T)alpha_d;
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[1] = {0.0};
         vec_chunk(S.n, c.G, b, [&](int i) {
             const T si = v.r[i] - alpha * v.v[i];
@@ -1263,9 +1326,9 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
             v.sh[i] = si / v.d[i];
             acc[0] += (double)si * (double)si;
         });
-        store_partials<1>(acc, S.part + 8 * (size_t)c.G, c.G, b);
+        store_partials<1>(acc, gl(S.part) + 8 * (size_t)c.G, c.G, b);
     }
-    if (c.b0 == 0 && threadIdx.x == 0) {
+    if (c.b == 0 && threadIdx.x == 0) {
         st->alpha = alpha_d;
         st->rho0 = rho1;
     }
@@ -1278,31 +1341,31 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv_t(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
     double ss[1];
-    reduce_partials<1>(S.part + 8 * (size_t)c.G, c.G, ss);
+    reduce_partials<1>(gl(S.part) + 8 * (size_t)c.G, c.G, ss);
     const double err = sqrt(ss[0]) / st->bnorm;
-    const KVec<T> v = kv_of<T>(S);
-    if (err < S.prm->tol) {
+    const GVec<T> v = gvec(kv_of<T>(S));
+    if (err < gl(S.prm)->tol) {
         const T alpha = (T)st->alpha;
         vec_vblocks(c, S.n, [&](int i) { v.x[i] += alpha * v.ph[i]; });
-        if (c.b0 == 0 && threadIdx.x == 0) {
+        if (c.b == 0 && threadIdx.x == 0) {
             st->error = err;
             st->pend = 1;
         }
         return;
     }
-    const MatView<VT> m = mv_of<VT>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    {
+        const int b = c.b;
         double acc[2] = {0.0, 0.0};
         spmv_rows<FMT, VT, T, T>(m, v.sh, c.G, b, [&](int i, T ax) {
             v.t[i] = ax;
             acc[0] += (double)ax * (double)ax;
             acc[1] += (double)ax * (double)v.s[i];
         });
-        store_partials<2>(acc, S.part + 12 * (size_t)c.G, c.G, b);
+        store_partials<2>(acc, gl(S.part) + 12 * (size_t)c.G, c.G, b);
     }
 }
 
@@ -1313,16 +1376,16 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    KrylovState* st = S.st;
+    OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
     double sv[2];
-    reduce_partials<2>(S.part + 12 * (size_t)c.G, c.G, sv);
+    reduce_partials<2>(gl(S.part) + 12 * (size_t)c.G, c.G, sv);
     const double omega_d = (sv[0] > DBL_MIN) ? sv[1] / sv[0] : 1.0;
     const T omega = (T)omega_d;
     const T alpha = (T)st->alpha;
-    const KVec<T> v = kv_of<T>(S);
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    {
+        const int b = c.b;
         double acc[2] = {0.0, 0.0};
         vec_chunk(S.n, c.G, b, [&](int i) {
             v.x[i] += alpha * v.ph[i] + omega * v.sh[i];
@@ -1331,9 +1394,9 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
             acc[0] += (double)ri * (double)ri;
             acc[1] += (double)v.r0[i] * (double)ri;
         });
-        store_partials<2>(acc, S.part, c.G, b);
+        store_partials<2>(acc, gl(S.part), c.G, b);
     }
-    if (c.b0 == 0 && threadIdx.x == 0) st->omega = omega_d;
+    if (c.b == 0 && threadIdx.x == 0) st->omega = omega_d;
 }
 
 // ---- FP32-inside-FP64 iterative refinement helpers
@@ -1344,11 +1407,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    const MatView<double> m = S.m64;
-    const KVec<double> v = S.v64;
-    double* r = S.r64;
-    for (int vb = 0; vb < c.nb; vb++) {
-        const int b = c.b0 + vb;
+    const GMat<double> m = gmat(S.m64);
+    const GVec<double> v = gvec(S.v64);
+    OGSX_GL double* r = gl(S.r64);
+    {
+        const int b = c.b;
         double acc[2] = {0.0, 0.0};
         spmv_rows<FMT, double, double, double>(m, v.x, c.G, b, [&](int i, double ax) {
             const double ri = v.b[i] - ax;
@@ -1356,7 +1419,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
             acc[0] += v.b[i] * v.b[i];
             acc[1] += ri * ri;
         });
-        store_partials<2>(acc, S.part, c.G, b);
+        store_partials<2>(acc, gl(S.part), c.G, b);
     }
 }
 
@@ -1366,9 +1429,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_fold(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    if (!S.irs->started) return;
-    const double u = S.prm->unscale;
-    vec_vblocks(c, S.n, [&](int i) { S.v64.x[i] += u * (double)S.v32.x[i]; });
+    if (!gl(S.irs)->started) return;
+    const double u = gl(S.prm)->unscale;
+    OGSX_GL double* x = gl(S.v64.x);
+    const OGSX_GL float* x32 = gl(S.v32.x);
+    vec_vblocks(c, S.n, [&](int i) { x[i] += u * (double)x32[i]; });
 }
 
 // One block per listed system: after k_ir_residual, the refinement step.
@@ -1382,11 +1447,11 @@ __global__ void k_ir_decide(Grp g)
 {
     const SysDev& S = *sys_of_block(g);
     double s[2];
-    reduce_partials<2>(S.part, S.G, s);
+    reduce_partials<2>(gl(S.part), S.G, s);
     if (threadIdx.x != 0) return;
-    IRState& R = *S.irs;
-    KrylovParams& P = *S.prm;
-    KrylovState& st = *S.st;
+    OGSX_GL IRState& R = *gl(S.irs);
+    OGSX_GL KrylovParams& P = *gl(S.prm);
+    OGSX_GL KrylovState& st = *gl(S.st);
     R.action = 0;
     // a finished solve leaves its recurrence stopped, so batched launches skip it
     auto finish = [&](int status) {
@@ -1458,14 +1523,23 @@ __global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    const int act = S.irs->action;
+    const int act = gl(S.irs)->action;
     if (act == 0) return;
-    const double scale = S.prm->scale;
-    float* dst = act == 1 ? S.v32.b : S.v32.r;
+    const double scale = gl(S.prm)->scale;
+    OGSX_GL float* dst = gl(act == 1 ? S.v32.b : S.v32.r);
+    OGSX_GL float* x32 = gl(S.v32.x);
+    const OGSX_GL double* r64 = gl(S.r64);
     vec_vblocks(c, S.n, [&](int i) {
-        dst[i] = (float)(S.r64[i] * scale);
-        S.v32.x[i] = 0.0f;
+        dst[i] = (float)(r64[i] * scale);
+        x32[i] = 0.0f;
     });
+}
+
+// Sets the recurrence state of one system (tuning and timing runs of the
+// solve's SpMV kernel need it "running"; afterwards it goes back to stopped).
+__global__ void k_set_done(KrylovState* st, int done)
+{
+    if (threadIdx.x == 0) gl(st)->done = done;
 }
 
 // ------------------------------------------------------------------------
