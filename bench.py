@@ -125,8 +125,9 @@ def cpu_baseline(iters_per_solve, n_target, nnz_target, e_target, n_solves):
 
 def spmv_mode(size):
     """Short run for hardware-counter passes: a 1 GiB copy (calibration of
-    the byte counters) and 20 SpMV launches each of the FP64 and FP32 matrix
-    of the assembled steady 128^3 system."""
+    the byte counters) and 20 launches each of the solve's SpMV kernel
+    (k_cg_spmv, tuned format) on the FP64 and FP32 matrix of the assembled
+    steady 128^3 system."""
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
     sc = W.hex_flow(size, n_transient=0)
@@ -143,7 +144,8 @@ def spmv_mode(size):
     print(json.dumps({"copy_1GiB_gbs": copy, "spmv64": {"avg_ms": ms64, "bytes": b64,
                                                           "gbs": b64 / ms64 / 1e6},
                       "spmv32": {"avg_ms": ms32, "bytes": b32, "gbs": b32 / ms32 / 1e6},
-                      "variant64": s.spmv_variant(), "tuning": s.spmv_tuning()}))
+                      "format64": s.spmv_format(False), "format32": s.spmv_format(True),
+                      "tuning": s.spmv_tuning()}))
 
 
 def _log(msg):
@@ -211,7 +213,7 @@ def ensemble_mode(count, steps):
     }))
 
 
-def serve_runs(count, rate, plain=True):
+def serve_runs(count, rate, plain=True, fp64=True):
     """Served stream: `count` requests, Poisson arrivals at `rate`/s, 2-D
     quads 32^2-512^2 and 3-D hexes 16^3-96^3, 1-20 steps each.  The
     continuous-batching server in FP32-IR (and FP64), and the plain method
@@ -221,7 +223,7 @@ def serve_runs(count, rate, plain=True):
     from ogs5_amd.serve import Server, serve_sequential
     specs = W.request_stream(count, rate)
     res = {}
-    runs = [("batched_fp32_ir", L.FP32_IR), ("batched_fp64", L.FP64)]
+    runs = [("batched_fp32_ir", L.FP32_IR)] + ([("batched_fp64", L.FP64)] if fp64 else [])
     for name, prec in runs:
         out = Server(prec).serve(specs)
         out.pop("jobs")
@@ -298,11 +300,15 @@ def main():
                 kernel=("k_cg_spmv<float>" if best is ir else "k_cg_spmv<double>"),
                 avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"],
                 format=best["spmv_format"])
+    # HBM bytes per dispatch of this kernel and format, from the FETCH_SIZE /
+    # WRITE_SIZE passes over `bench.py --mode spmv` (profiles/spmv_hex128_traffic.md)
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
-        tr = json.load(open(tf)).get(roof["kernel"])
+        key = f"k_cg_spmv<{best['spmv_format']},{'float' if best is ir else 'double'}>"
+        tr = json.load(open(tf)).get(key)
         if tr:
             roof["traffic"] = tr
+            roof["traffic_source"] = key
     copy_gbs = ox._lib.stream_copy_gbs(1 << 30, 10)
     roof["copy_kernel_gbs"] = copy_gbs
     cpu = None
@@ -334,7 +340,7 @@ def main():
         "served_p99_ms": None,
     }
     if args.serve_requests > 0:
-        sv = serve_runs(args.serve_requests, args.rate, plain=False)["batched_fp32_ir"]
+        sv = serve_runs(args.serve_requests, args.rate, plain=False, fp64=False)["batched_fp32_ir"]
         out["served_p50_ms"] = sv["latency_ms"]["p50"]
         out["served_p99_ms"] = sv["latency_ms"]["p99"]
         out["served"] = dict(requests=sv["requests"], rate_per_s=args.rate,
