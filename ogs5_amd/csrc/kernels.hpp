@@ -302,10 +302,17 @@ __device__ __forceinline__ void spmv_tiles(const GMat<VT>& m, const OGSX_GL XT* 
 // instruction), rows need no cross-lane reduction, and for meshes numbered
 // along a grid line the 64 lanes' x gathers of one entry j touch 64
 // consecutive x values.  Blocks take contiguous slice ranges, XCD-aware.
-template <typename P>
-__device__ __forceinline__ auto ld(P p, bool nt)
+// NT selects non-temporal loads (the matrix streams through once per SpMV;
+// keeping it out of the caches leaves them to x and the Krylov vectors).  A
+// compile-time choice: a run-time select between the two loads of the same
+// address gets merged into one plain load, losing the hint.
+template <bool NT, typename P>
+__device__ __forceinline__ auto ld(P p)
 {
-    return nt ? __builtin_nontemporal_load(p) : *p;
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
 }
 
 template <int C16, int U, bool NT, typename VT, typename XT, typename ACC, typename Epi>
@@ -328,8 +335,11 @@ __device__ __forceinline__ void spmv_sell(const GMat<VT>& m, const OGSX_GL XT* _
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int k = base + (j + u) * kSellC;
-                c[u] = C16 ? cb + (int)ld(m.sell_c16 + k, NT) : ld(m.sell_c32 + k, NT);
-                v[u] = ld(m.sval + k, NT);
+                if constexpr (C16)
+                    c[u] = cb + (int)ld<NT>(m.sell_c16 + k);
+                else
+                    c[u] = ld<NT>(m.sell_c32 + k);
+                v[u] = ld<NT>(m.sval + k);
             }
             ACC xv[U];
 #pragma unroll
