@@ -265,7 +265,7 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
 #define OGSX_PROF_SPMV32 1  /* FP32 SpMV kernel (IR inner solve)             */
 #define OGSX_PROF_ASSEMBLY 2
 /* SpMV storage/launch variant: 0 = autotune (time variants 1, 2, 3, 5, 6, 7,
- * 10, 11 on this matrix at first use and keep the fastest; all of them sum each row in
+ * 10, 11, 12 on this matrix at first use and keep the fastest; all of them sum each row in
  * CRS order over rounded products, bit-identical to CSparseMatrix::multiVec,
  * so the choice never changes results); 1 = tiled CRS (blocks stream
  * contiguous nonzero tiles through LDS); 2 = tiled CRS with 16-bit column
@@ -274,7 +274,7 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * per row); 5 = SELL-64 with 16-bit column offsets per slice; 6 = as 5 with
  * 8 entries in flight per lane; 7 = as 6 with non-temporal matrix loads;
  * 10 = 16 entries in flight, non-temporal; 11 = two slices per wavefront,
- * non-temporal; 4, 8, 16, 32,
+ * non-temporal; 12 = 7 entries in flight, non-temporal; 4, 8, 16, 32,
  * 64 = vector CRS with that many lanes per row (lane-parallel partial sums:
  * a different rounding order).  The CRS arrays stay the system's matrix;
  * SELL copies are refreshed after A changes. */

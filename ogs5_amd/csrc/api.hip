@@ -217,7 +217,7 @@ bool valid_fmt(const ogsx_system* s, int f)
     if (f == kFmtTile32 || f == kFmtSell32) return true;
     if (f == kFmtTile16) return s->tile16;
     if (f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT || f == kFmtSell16U16 ||
-        f == kFmtSell16X2)
+        f == kFmtSell16X2 || f == kFmtSell16U7)
         return s->sell16;
     return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
 }
@@ -238,7 +238,7 @@ MatView<VT> mat_view(const ogsx_system* s, const VT* val)
 inline bool is_sell(int f)
 {
     return f == kFmtSell32 || f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT ||
-           f == kFmtSell16U16 || f == kFmtSell16X2;
+           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7;
 }
 
 // SELL-64 layout of the pattern: slice widths, offsets, columns (32-bit and,
@@ -437,6 +437,7 @@ int prof_flush(ogsx_system* s)
         case kFmtSell16NT: { constexpr int L = kFmtSell16NT; CALL; } break; \
         case kFmtSell16U16: { constexpr int L = kFmtSell16U16; CALL; } break; \
         case kFmtSell16X2: { constexpr int L = kFmtSell16X2; CALL; } break; \
+        case kFmtSell16U7: { constexpr int L = kFmtSell16U7; CALL; } break; \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -468,7 +469,8 @@ template <typename VT, typename T>
 int tune_one(ogsx_system* s, int* fmt_out)
 {
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
-                         kFmtSell16X2,  kFmtSell32,   kFmtTile16,   kFmtTile32};
+                         kFmtSell16X2,  kFmtSell16U7, kFmtSell32,   kFmtTile16,
+                         kFmtTile32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -515,7 +517,8 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
 {
     const double n = s->n, nnz = (double)s->nnz;
     const int sidx = (fmt == kFmtTile16 || fmt == kFmtSell16 || fmt == kFmtSell16U8 ||
-                      fmt == kFmtSell16NT || fmt == kFmtSell16U16 || fmt == kFmtSell16X2)
+                      fmt == kFmtSell16NT || fmt == kFmtSell16U16 || fmt == kFmtSell16X2 ||
+                      fmt == kFmtSell16U7)
                          ? 2
                          : 4;
     if (is_sell(fmt))  // stored entries incl. padding, slice table (ptr, width, base)
@@ -1312,7 +1315,7 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
     }
     if (!valid_fmt(s, lanes_per_row))
         return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
-                                 "3/5/6/7/10/11 (SELL-64), or 4, 8, 16, 32, 64 lanes per row "
+                                 "3/5/6/7/10/11/12 (SELL-64), or 4, 8, 16, 32, 64 lanes per row "
                                  "(16-bit forms only when the pattern allows)");
     s->fmt64 = s->fmt32 = lanes_per_row;
     return OGSX_OK;

@@ -216,6 +216,7 @@ constexpr int kFmtSell16U8 = 6;           // ... 8 entries in flight per lane
 constexpr int kFmtSell16NT = 7;           // ... and non-temporal matrix loads
 constexpr int kFmtSell16U16 = 10;         // SELL-64, 16-bit, 16 in flight, NT
 constexpr int kFmtSell16X2 = 11;          // SELL-64, 16-bit, two slices per wave, NT
+constexpr int kFmtSell16U7 = 12;          // SELL-64, 16-bit, 7 entries in flight, NT
 constexpr int kSellC = 64;                // rows per slice = wavefront size
 
 // Tiled CRS SpMV: a block streams the contiguous nonzeros of one tile of
@@ -442,6 +443,8 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
         spmv_sell<1, 8, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16U16)
         spmv_sell<1, 16, true, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSell16U7)
+        spmv_sell<1, 7, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16X2)
         spmv_sell_x2<4, VT, XT, ACC>(m, x, G, b, epi);
     else

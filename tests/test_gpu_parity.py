@@ -96,7 +96,7 @@ def test_assembly_parity(name, bc):
     assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
 
 
-@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 4, 8, 16, 32, 64])
 def test_spmv_parity_all_variants(lpr):
     sc = SCEN["hex"]
     A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_ROW)
@@ -106,7 +106,7 @@ def test_spmv_parity_all_variants(lpr):
     assert s.spmv_variant() == lpr
     eq.a[:] = A  # same matrix on both sides
     yr = eq.spmv(x)
-    if lpr in (1, 2, 3, 5, 6, 7, 10, 11):
+    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12):
         # CRS-order summation over rounded products: the reference's multiVec bits
         assert np.array_equal(y, yr)
         return
