@@ -10,6 +10,7 @@
 #include <climits>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -125,6 +126,8 @@ struct ogsx_system {
     bool tile16 = false;  // every tile's columns fit 16-bit offsets
     int *tile_row = nullptr, *tile_cb = nullptr;
     uint16_t* col16 = nullptr;
+    bool extra_layouts = false;  // tiles and 32-bit SELL columns on the device
+    const PatternLayout* lay = nullptr;
     std::vector<std::pair<int, double>> tune_log;  // (format, ms) of the last tuning
     // SELL-64 copy of the matrix (values refreshed from A when A changed)
     int n_slices = 0;
@@ -192,12 +195,35 @@ struct ogsx_system {
 
 namespace {
 
+// Systems allocate from the device's stream-ordered memory pool when the
+// device has one (freed memory stays in the pool: creating and destroying
+// systems in a serving loop then costs no device-wide synchronisation and
+// no trip to the driver); plain hipMalloc otherwise.
+bool use_pool()
+{
+    static std::once_flag once;
+    static bool pool = false;
+    std::call_once(once, [] {
+        int dev = 0, sup = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&sup, hipDeviceAttributeMemoryPoolsSupported, dev) != hipSuccess ||
+            !sup)
+            return;
+        hipMemPool_t mp;
+        if (hipDeviceGetDefaultMemPool(&mp, dev) != hipSuccess) return;
+        uint64_t keep = UINT64_MAX;
+        pool = hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+    });
+    return pool;
+}
+
 template <typename T>
 int dalloc(ogsx_system* s, T** p, size_t count)
 {
     if (count == 0) count = 1;
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    hipError_t e = use_pool() ? hipMallocAsync(&q, count * sizeof(T), s->stream)
+                              : hipMalloc(&q, count * sizeof(T));
     if (e != hipSuccess)
         return fail(OGSX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
     s->allocs.push_back(q);
@@ -220,6 +246,12 @@ bool valid_fmt(const ogsx_system* s, int f)
         f == kFmtSell16X2 || f == kFmtSell16U7)
         return s->sell16;
     return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
+}
+
+// Formats that read the row tiles or the 32-bit SELL columns.
+inline bool needs_extra_layouts(int f)
+{
+    return f == kFmtTile32 || f == kFmtTile16 || f == kFmtSell32;
 }
 
 template <typename VT>
@@ -370,14 +402,15 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
     auto b = [](int64_t count, int64_t size) { return std::max<int64_t>(count, 1) * size; };
     const int ed = edim_of(h.etype);
     int64_t t = b(n + 1, 4) + b(nnz, 4) + b(n, 4) + b((int64_t)h.conn.size(), 4) +
-                b((int64_t)h.color_elems.size(), 4) + b((int64_t)h.slot.size(), 4) + b(nnz, 8) +
+                b((int64_t)h.color_elems.size(), 4) + b((int64_t)h.conn.size() * h.npe, 4) + b(nnz, 8) +
                 b(n, 8) + 12 * b(n, 8) + b(16 * G, 8) + b(8, 8) + b(1, sizeof(KrylovState)) +
                 b(n, 1) + b(n, 8) + b(1, sizeof(SysDev)) + b(1, sizeof(KrylovParams));
     if (precision == OGSX_FP32_IR) t += b(nnz, 4) + 12 * b(n, 4) + b(n, 8) + b(1, sizeof(IRState));
-    t += b((int64_t)L.trow.size(), 4) + b((int64_t)L.tcb.size(), 4) +
-         (L.tile16 ? b((int64_t)L.tc16.size(), 2) : 0);
+    if (!L.sell16)  // tiles and 32-bit SELL columns, built at creation only then
+        t += b((int64_t)L.trow.size(), 4) + b((int64_t)L.tcb.size(), 4) +
+             (L.tile16 ? b((int64_t)L.tc16.size(), 2) : 0) + b((int64_t)L.sc32.size(), 4);
     t += b((int64_t)L.sptr.size(), 4) + b((int64_t)L.sw.size(), 4) + b((int64_t)L.scb.size(), 4) +
-         b((int64_t)L.sc32.size(), 4) + b((int64_t)L.ssrc.size(), 4) + b(L.npad, 8) +
+         b((int64_t)L.ssrc.size(), 4) + b(L.npad, 8) +
          (precision == OGSX_FP32_IR ? b(L.npad, 4) : 0) + (L.sell16 ? b((int64_t)L.sc16.size(), 2) : 0);
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
@@ -481,7 +514,7 @@ int tune_one(ogsx_system* s, int* fmt_out)
     float best_ms = 1e30f;
     const int reps = 4;
     for (int f : cands) {
-        if (!valid_fmt(s, f)) continue;
+        if (!valid_fmt(s, f) || (needs_extra_layouts(f) && !s->extra_layouts)) continue;
         launch_solve_spmv<VT, T>(s, f);
         (void)hipEventRecord(e0, s->stream);
         for (int r = 0; r < reps; r++) launch_solve_spmv<VT, T>(s, f);
@@ -525,6 +558,28 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
         return (double)s->npad * (sval + sidx) + 12.0 * s->n_slices + 2 * n * sx;
     const double tiles = (fmt == kFmtTile16 || fmt == kFmtTile32) ? 8.0 * s->n_tiles : 0.0;
     return nnz * (sval + sidx) + (n + 1) * 4 + 2 * n * sx + tiles;
+}
+
+// Row tiles (formats 1, 2) and 32-bit SELL columns (format 3): allocated and
+// uploaded on first need (the caller synchronises the stream).
+int alloc_extra_layouts(ogsx_system* s)
+{
+    if (s->extra_layouts) return OGSX_OK;
+    const PatternLayout& Ly = *s->lay;
+    OK_TRY(dalloc(s, &s->tile_row, Ly.trow.size()));
+    OK_TRY(dalloc(s, &s->tile_cb, Ly.tcb.size()));
+    if (s->tile16) OK_TRY(dalloc(s, &s->col16, Ly.tc16.size()));
+    OK_TRY(dalloc(s, &s->sell_c32, Ly.sc32.size()));
+    auto up = [&](void* d, const void* src, size_t bytes) {
+        return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s->stream);
+    };
+    if (up(s->tile_row, Ly.trow.data(), 4 * Ly.trow.size()) ||
+        up(s->tile_cb, Ly.tcb.data(), 4 * Ly.tcb.size()) ||
+        (s->tile16 && up(s->col16, Ly.tc16.data(), 2 * Ly.tc16.size())) ||
+        up(s->sell_c32, Ly.sc32.data(), 4 * Ly.sc32.size()))
+        return fail(OGSX_EHIP, "upload of the tiled / 32-bit SELL layouts failed");
+    s->extra_layouts = true;
+    return OGSX_OK;
 }
 
 template <typename T>
@@ -628,7 +683,12 @@ int ogsx_system_destroy(ogsx_system* s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     delete s->solo;
     if (s->ready) (void)hipEventDestroy(s->ready);
-    for (void* q : s->allocs) (void)hipFree(q);
+    if (use_pool() && s->stream) {
+        for (void* q : s->allocs) (void)hipFreeAsync(q, s->stream);
+        (void)hipStreamSynchronize(s->stream);  // back in the pool before the stream goes
+    } else {
+        for (void* q : s->allocs) (void)hipFree(q);
+    }
     if (s->prm_host) (void)hipHostFree(s->prm_host);
     if (s->irs_host) (void)hipHostFree(s->irs_host);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
@@ -665,7 +725,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     if ((rc = dalloc(s, &s->ptr, h.n + 1)) || (rc = dalloc(s, &s->col, h.nnz)) ||
         (rc = dalloc(s, &s->diagpos, h.n)) || (rc = dalloc(s, &s->conn, h.conn.size())) ||
         (rc = dalloc(s, &s->color_elems, h.color_elems.size())) ||
-        (rc = dalloc(s, &s->slot, h.slot.size())) || (rc = dalloc(s, &s->A, h.nnz)) ||
+        (rc = dalloc(s, &s->slot, h.conn.size() * h.npe)) || (rc = dalloc(s, &s->A, h.nnz)) ||
         (rc = dalloc(s, &s->uold, h.n)) || (rc = alloc_vecs(s, s->v64)) ||
         (rc = dalloc(s, &s->part, (size_t)16 * s->G)) || (rc = dalloc(s, &s->red, 8)) ||
         (rc = dalloc(s, &s->st, 1)) || (rc = dalloc(s, &s->bcflag, h.n)) ||
@@ -691,9 +751,11 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     if (up(s->ptr, Ly.ptr32.data(), 4 * (h.n + 1)) || up(s->col, Ly.col32.data(), 4 * h.nnz) ||
         up(s->diagpos, Ly.dg32.data(), 4 * h.n) ||
         up(s->conn, h.conn.data(), 4 * h.conn.size()) ||
-        up(s->color_elems, h.color_elems.data(), 4 * h.color_elems.size()) ||
-        up(s->slot, h.slot.data(), 4 * h.slot.size()))
+        up(s->color_elems, h.color_elems.data(), 4 * h.color_elems.size()))
         return cleanup(fail(OGSX_EHIP, "upload of the pattern failed"));
+    if (!h.conn.empty())
+        k_slot_map<<<blocks_for((int64_t)h.conn.size(), kBlock), kBlock, 0, s->stream>>>(
+            (int64_t)h.conn.size(), h.npe, s->conn, s->ptr, s->col, s->slot);
     const int nb = blocks_for(h.nnz, kBlock * 4);
     k_fill<double><<<nb, kBlock, 0, s->stream>>>(h.nnz, s->A, 0.0);
     double* zv[] = {s->v64.x, s->v64.b, s->uold};
@@ -701,34 +763,30 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     k_fill<int8_t><<<blocks_for(h.n, kBlock * 4), kBlock, 0, s->stream>>>(h.n, s->bcflag, (int8_t)0);
     if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess)
         return cleanup(fail(OGSX_EHIP, "initialisation kernels failed"));
+    s->lay = &p->L;
+    s->tile16 = Ly.tile16;
+    s->n_tiles = (int)Ly.trow.size() - 1;
+    s->sell16 = Ly.sell16;
+    s->n_slices = (int)Ly.sw.size();
+    s->npad = Ly.npad;
     {
-        s->tile16 = Ly.tile16;
-        s->n_tiles = (int)Ly.trow.size() - 1;
-        if ((rc = dalloc(s, &s->tile_row, Ly.trow.size())) || (rc = dalloc(s, &s->tile_cb, Ly.tcb.size())))
-            return cleanup(rc);
-        if (s->tile16 && (rc = dalloc(s, &s->col16, Ly.tc16.size()))) return cleanup(rc);
-        if (up(s->tile_row, Ly.trow.data(), 4 * Ly.trow.size()) ||
-            up(s->tile_cb, Ly.tcb.data(), 4 * Ly.tcb.size()) ||
-            (s->tile16 && up(s->col16, Ly.tc16.data(), 2 * Ly.tc16.size())))
-            return cleanup(fail(OGSX_EHIP, "upload of the tiles failed"));
-    }
-    {
-        s->sell16 = Ly.sell16;
-        s->n_slices = (int)Ly.sw.size();
-        s->npad = Ly.npad;
         if ((rc = dalloc(s, &s->sell_ptr, Ly.sptr.size())) || (rc = dalloc(s, &s->sell_w, Ly.sw.size())) ||
-            (rc = dalloc(s, &s->sell_cb, Ly.scb.size())) || (rc = dalloc(s, &s->sell_c32, Ly.sc32.size())) ||
+            (rc = dalloc(s, &s->sell_cb, Ly.scb.size())) ||
             (rc = dalloc(s, &s->sell_src, Ly.ssrc.size())) || (rc = dalloc(s, &s->sval64, s->npad)))
             return cleanup(rc);
         if (precision == OGSX_FP32_IR && (rc = dalloc(s, &s->sval32, s->npad))) return cleanup(rc);
         if (s->sell16 && (rc = dalloc(s, &s->sell_c16, Ly.sc16.size()))) return cleanup(rc);
         if (up(s->sell_ptr, Ly.sptr.data(), 4 * Ly.sptr.size()) || up(s->sell_w, Ly.sw.data(), 4 * Ly.sw.size()) ||
-            up(s->sell_cb, Ly.scb.data(), 4 * Ly.scb.size()) || up(s->sell_c32, Ly.sc32.data(), 4 * Ly.sc32.size()) ||
+            up(s->sell_cb, Ly.scb.data(), 4 * Ly.scb.size()) ||
             up(s->sell_src, Ly.ssrc.data(), 4 * Ly.ssrc.size()) ||
-            (s->sell16 && up(s->sell_c16, Ly.sc16.data(), 2 * Ly.sc16.size())) ||
-            hipStreamSynchronize(s->stream))
+            (s->sell16 && up(s->sell_c16, Ly.sc16.data(), 2 * Ly.sc16.size())))
             return cleanup(fail(OGSX_EHIP, "upload of the SELL layout failed"));
     }
+    // the 32-bit-column SELL and tiled layouts are tuner candidates only when
+    // 16-bit slice offsets do not fit; otherwise they are built on request
+    if (!s->sell16 && (rc = alloc_extra_layouts(s))) return cleanup(rc);
+    if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return cleanup(fail(OGSX_EHIP, "upload of the SpMV layouts failed"));
     {
         SysDev& d = s->devh;
         d.m64 = mat_view(s, (const double*)s->A);
@@ -1317,6 +1375,17 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
         return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
                                  "3/5/6/7/10/11/12 (SELL-64), or 4, 8, 16, 32, 64 lanes per row "
                                  "(16-bit forms only when the pattern allows)");
+    if (needs_extra_layouts(lanes_per_row) && !s->extra_layouts) {
+        OK_TRY(alloc_extra_layouts(s));
+        // the matrix views of the device descriptor now hold the new arrays
+        s->devh.m64 = mat_view(s, (const double*)s->A);
+        s->devh.m32 = mat_view(s, (const float*)s->A32);
+        HIP_TRY(hipMemcpyAsync(s->dev, &s->devh, sizeof(SysDev), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        delete s->solo;
+        s->solo = nullptr;
+        if (s->group) OK_TRY(set_members(s->group->set, s->group->set->mem));
+    }
     s->fmt64 = s->fmt32 = lanes_per_row;
     return OGSX_OK;
 }

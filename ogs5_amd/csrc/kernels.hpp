@@ -1548,6 +1548,33 @@ __global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g)
     });
 }
 
+// Element -> CRS slot map, built on the device from the uploaded pattern:
+// slot[(e npe + i) npe + j] = position of column conn[e][j] in row conn[e][i]
+// (lower_bound in the sorted row; the pattern holds every pair of nodes that
+// share an element, so the column is found).
+__global__ __launch_bounds__(kBlock) void k_slot_map(int64_t ne_npe, int npe, const int* conn,
+                                                     const int* ptr, const int* col, int* slot)
+{
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < ne_npe;
+         t += (int64_t)gridDim.x * kBlock) {
+        const int64_t e = t / npe;
+        const int r = conn[t];
+        const int lo0 = ptr[r], hi0 = ptr[r + 1];
+        for (int j = 0; j < npe; j++) {
+            const int c = conn[e * npe + j];
+            int lo = lo0, hi = hi0;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (col[mid] < c)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            slot[t * npe + j] = min(lo, hi0 - 1);
+        }
+    }
+}
+
 // Sets the recurrence state of one system (tuning and timing runs of the
 // solve's SpMV kernel need it "running"; afterwards it goes back to stopped).
 __global__ void k_set_done(KrylovState* st, int done)

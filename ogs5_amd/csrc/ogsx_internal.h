@@ -47,8 +47,8 @@ inline int ngp_of(int et, int ng)
     return 0;
 }
 
-// Host-side pattern: CRS in the reference's layout plus the colouring and
-// the element -> slot map of the atomic-free scatter.
+// Host-side pattern: CRS in the reference's layout plus the element
+// colouring of the atomic-free scatter.
 struct PatternHost {
     int64_t n = 0, nnz = 0, n_elem = 0;
     int etype = 0, npe = 0;
@@ -57,7 +57,6 @@ struct PatternHost {
     std::vector<int32_t> color;            // [n_elem]
     std::vector<int32_t> color_ptr;        // [n_colors+1]
     std::vector<int32_t> color_elems;      // [n_elem] element ids by colour
-    std::vector<int32_t> slot;             // [n_elem*npe*npe]
     int n_colors = 0;
 };
 

@@ -1,5 +1,5 @@
 // pattern.cpp -- host construction of the CRS sparsity pattern, the element
-// colouring and the element -> CRS slot map.
+// colouring (the element -> CRS slot map is built on the device).
 //
 // The CRS arrays reproduce the reference bit for bit:
 //   CFEMesh::ConnectedNodes(false)  MSH/msh_mesh.cpp:2931-2979: for every node,
@@ -129,22 +129,6 @@ int build_pattern(int64_t n_nodes, int64_t n_elem, int etype,
             for (int32_t c : buf) {
                 if (c == i) P.diag[i] = w;
                 P.col[w++] = c;
-            }
-        }
-    });
-    // element -> CRS slot map (binary search of column j in row i)
-    P.slot.resize((size_t)n_elem * npe * npe);
-    parallel_for(n_elem, [&](int64_t b, int64_t e) {
-        for (int64_t el = b; el < e; el++) {
-            const int32_t* en = &P.conn[el * npe];
-            for (int i = 0; i < npe; i++) {
-                const int64_t r = en[i];
-                const int64_t* c0 = P.col.data() + P.ptr[r];
-                const int64_t* c1 = P.col.data() + P.ptr[r + 1];
-                for (int j = 0; j < npe; j++) {
-                    const int64_t* f = std::lower_bound(c0, c1, (int64_t)en[j]);
-                    P.slot[(el * npe + i) * npe + j] = (int32_t)(f - P.col.data());
-                }
             }
         }
     });

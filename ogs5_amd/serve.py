@@ -8,7 +8,8 @@ request.  A request that arrives while others are on the GPU is admitted
 as soon as a slot and its HBM footprint (ogsx_pattern_footprint) fit, and
 its solves join the running batch between two iteration batches.  Host
 work of a request (mesh, K field, CRS pattern, colouring and device
-layouts) runs on a worker pool, off the GPU loop.
+layouts) runs on a worker pool, and its device set-up and tear-down on one
+more thread, off the GPU loop.
 
 Each step of a request is OGS-5's time step on its own system: Initialize
 -> assembly -> source terms -> Dirichlet -> solve -> copy-back
@@ -55,6 +56,24 @@ def _setup(spec: W.RequestSpec, precision: int, t0: float) -> Job:
     return Job(spec, sc, pat, pat.footprint(precision), time.perf_counter() - t0)
 
 
+def _materialize(job: Job, precision: int) -> Job:
+    """Device set-up of an admitted request: system, element data, initial
+    condition (on the request's own stream, off the GPU loop's thread)."""
+    s = L.System(job.pattern, precision)
+    s.set_elements(**job.sc.element_kwargs())
+    s.set_vector(L.VEC_UOLD, job.sc.u0)
+    s.set_vector(L.VEC_X, job.sc.u0)
+    s.synchronize()
+    job.system = s
+    return job
+
+
+def _teardown(job: Job):
+    job.system.close()
+    job.system = None
+    job.pattern.close()
+
+
 class Server:
     """Continuous-batching solver of flow requests on one GPU.
 
@@ -76,10 +95,15 @@ class Server:
         self.tol = tol
         self.max_iter = max_iter
 
-    def _begin_step(self, group, job: Job):
-        sc, s = job.sc, job.system
-        W._prepare_step(sc, s, job.step, L.DIRICHLET_SYMMETRIC)
-        group.start([s], method=L.CG, tol=self.tol, max_iter=self.max_iter)
+    def _begin_steps(self, group, jobs: List[Job]):
+        """Assembles each job's next step on its own stream, then starts all
+        their solves with one group call (one set of launches for all)."""
+        if not jobs:
+            return
+        for job in jobs:
+            W._prepare_step(job.sc, job.system, job.step, L.DIRICHLET_SYMMETRIC)
+        group.start([j.system for j in jobs], method=L.CG, tol=self.tol,
+                    max_iter=self.max_iter)
 
     def serve(self, specs: List[W.RequestSpec], keep_x: bool = False,
               log=None) -> dict:
@@ -89,9 +113,13 @@ class Server:
         group = L.Group(self.precision, self.use_graphs)
         order = sorted(specs, key=lambda s: s.arrival)
         pool = ThreadPoolExecutor(self.workers)
+        # creates admitted requests' systems and destroys finished ones while
+        # the loop keeps the group running
+        devq = ThreadPoolExecutor(1)
         t0 = time.perf_counter()
         setups = deque()  # futures in arrival order
         ready: deque = deque()
+        creating: deque = deque()  # futures of admitted requests being set up
         live: dict = {}
         done: List[Job] = []
         used = 0
@@ -108,32 +136,35 @@ class Server:
                 while setups and setups[0].done():
                     ready.append(setups.popleft().result())
                 # admission in arrival order, by slots and HBM footprint
-                while ready and len(live) < self.max_active and \
+                while ready and len(live) + len(creating) < self.max_active and \
                         used + ready[0].footprint <= self.hbm_budget:
                     job = ready.popleft()
-                    s = L.System(job.pattern, self.precision)
-                    s.set_elements(**job.sc.element_kwargs())
-                    s.set_vector(L.VEC_UOLD, job.sc.u0)
-                    s.set_vector(L.VEC_X, job.sc.u0)
-                    group.add(s)
-                    job.system = s
-                    job.t_admit = time.perf_counter() - t0
                     used += job.footprint
-                    live[job.spec.rid] = job
-                    self._begin_step(group, job)
-                if ready and not live and ready[0].footprint > self.hbm_budget:
+                    creating.append(devq.submit(_materialize, job, self.precision))
+                if ready and not live and not creating and ready[0].footprint > self.hbm_budget:
                     raise L.OgsxError(f"request {ready[0].spec.rid} needs "
                                       f"{ready[0].footprint} bytes, above the HBM budget")
+                joining = []
+                while creating and creating[0].done():
+                    job = creating.popleft().result()
+                    group.add(job.system)
+                    job.t_admit = time.perf_counter() - t0
+                    live[job.spec.rid] = job
+                    joining.append(job)
+                self._begin_steps(group, joining)
                 peak_used = max(peak_used, used)
                 peak_live = max(peak_live, len(live))
                 if not live:
-                    if nxt < len(order) and not setups and not ready:
+                    if creating:
+                        creating[0].result()
+                    elif nxt < len(order) and not setups and not ready:
                         time.sleep(max(0.0, min(order[nxt].arrival - now, 0.002)))
                     else:
                         time.sleep(0.0005)
                     continue
                 group.advance(self.batch_iters)
                 batches += 1
+                stepping = []
                 for rid in list(live):
                     job = live[rid]
                     active, info = L.solve_status(job.system)
@@ -144,7 +175,7 @@ class Server:
                     s.copy_vector(L.VEC_UOLD, L.VEC_X)
                     job.step += 1
                     if job.step < job.spec.steps:
-                        self._begin_step(group, job)
+                        stepping.append(job)
                         continue
                     if keep_x:
                         job.x = s.get_vector(L.VEC_X)
@@ -152,16 +183,16 @@ class Server:
                         s.synchronize()
                     job.t_done = time.perf_counter() - t0
                     group.remove(s)
-                    s.close()
-                    job.system = None
-                    job.pattern.close()
+                    devq.submit(_teardown, job)
                     used -= job.footprint
                     del live[rid]
                     done.append(job)
                     if log:
                         log(job)
+                self._begin_steps(group, stepping)
         finally:
             pool.shutdown(wait=True)
+            devq.shutdown(wait=True)
             group.close()
         return self._summary(done, time.perf_counter() - t0, peak_used, peak_live, batches)
 
