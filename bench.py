@@ -240,8 +240,11 @@ def serve_runs(count, rate, plain=True, fp64=True):
     return res
 
 
-def serve_mode(count, rate):
+def serve_mode(count, rate, count2, rate2):
     res = serve_runs(count, rate)
+    # the same kind of stream at a load one GPU sustains: latency is then the
+    # service time rather than the queue
+    res_sus = serve_runs(count2, rate2, fp64=False) if count2 > 0 else {}
     best = res["batched_fp32_ir"]
     print(json.dumps({
         "metric": "served p50/p99 latency; MDOF-solves/s to 1e-10 FP64 resid",
@@ -253,6 +256,7 @@ def serve_mode(count, rate):
                                "32^2-512^2 and hexes 16^3-96^3, 1-20 steps, Jacobi-PCG to 1e-10"},
         "served_p50_ms": best["latency_ms"]["p50"], "served_p99_ms": best["latency_ms"]["p99"],
         "runs": res,
+        "sustainable_load": dict(rate_per_s=rate2, requests=count2, runs=res_sus),
         "speedup_vs_plain_fp64": best["mdof_solves_per_s"] / res["plain_fp64"]["mdof_solves_per_s"],
     }))
 
@@ -269,6 +273,9 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--requests2", type=int, default=150,
+                    help="requests of the second served stream (--mode serve), at --rate2")
+    ap.add_argument("--rate2", type=float, default=15.0)
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests of the short served run in the full mode (0: none)")
     args = ap.parse_args()
@@ -283,7 +290,7 @@ def main():
         ensemble_mode(args.instances, args.ens_steps)
         return
     if args.mode == "serve":
-        serve_mode(args.requests, args.rate)
+        serve_mode(args.requests, args.rate, args.requests2, args.rate2)
         return
     import ogs5_amd as ox
     from ogs5_amd import workloads as W

@@ -22,9 +22,10 @@ its steps).
 """
 from __future__ import annotations
 
+import threading
 import time
-from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from collections import OrderedDict, deque
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -49,10 +50,41 @@ class Job:
     x: Optional[np.ndarray] = None
 
 
-def _setup(spec: W.RequestSpec, precision: int, t0: float) -> Job:
+class MeshCache:
+    """Meshes and CRS patterns shared by requests of the same kind and size
+    (both depend on nothing else): built once, by the first request that
+    needs them, and kept for the most recent `size` kinds/sizes.  Patterns
+    are read-only once built, so concurrent systems share one."""
+
+    def __init__(self, size: int = 32):
+        self.size = size
+        self.lock = threading.Lock()
+        self.entries: "OrderedDict[tuple, Future]" = OrderedDict()
+
+    def get(self, kind: str, side: int):
+        key = (kind, side)
+        with self.lock:
+            fut = self.entries.get(key)
+            mine = fut is None
+            if mine:
+                fut = Future()
+                self.entries[key] = fut
+                while len(self.entries) > self.size:
+                    self.entries.popitem(last=False)  # dropped patterns free with their last user
+            self.entries.move_to_end(key)
+        if mine:
+            try:
+                mesh = W.request_mesh(kind, side)
+                fut.set_result((mesh, L.Pattern(mesh.n_nodes, mesh.etype, mesh.conn)))
+            except BaseException as e:
+                fut.set_exception(e)
+        return fut.result()
+
+
+def _setup(spec: W.RequestSpec, precision: int, t0: float, cache: MeshCache) -> Job:
     """Host-side work of a request (runs on the worker pool; no GPU calls)."""
-    sc = W.request_scenario(spec)
-    pat = L.Pattern(sc.n, sc.mesh.etype, sc.mesh.conn)
+    mesh, pat = cache.get(spec.kind, spec.side)
+    sc = W.request_scenario(spec, mesh)
     return Job(spec, sc, pat, pat.footprint(precision), time.perf_counter() - t0)
 
 
@@ -71,7 +103,7 @@ def _materialize(job: Job, precision: int) -> Job:
 def _teardown(job: Job):
     job.system.close()
     job.system = None
-    job.pattern.close()
+    job.pattern = None  # shared through the mesh cache
 
 
 class Server:
@@ -116,6 +148,7 @@ class Server:
         # creates admitted requests' systems and destroys finished ones while
         # the loop keeps the group running
         devq = ThreadPoolExecutor(1)
+        cache = MeshCache()
         t0 = time.perf_counter()
         setups = deque()  # futures in arrival order
         ready: deque = deque()
@@ -131,7 +164,7 @@ class Server:
             while len(done) < len(order):
                 now = time.perf_counter() - t0
                 while nxt < len(order) and order[nxt].arrival <= now:
-                    setups.append(pool.submit(_setup, order[nxt], self.precision, t0))
+                    setups.append(pool.submit(_setup, order[nxt], self.precision, t0, cache))
                     nxt += 1
                 while setups and setups[0].done():
                     ready.append(setups.popleft().result())
@@ -227,6 +260,7 @@ def serve_sequential(specs: List[W.RequestSpec], precision: int = L.FP64,
     L.device_count()
     order = sorted(specs, key=lambda s: s.arrival)
     pool = ThreadPoolExecutor(workers)
+    cache = MeshCache()
     t0 = time.perf_counter()
     futs = {}
     done = []
@@ -239,10 +273,10 @@ def serve_sequential(specs: List[W.RequestSpec], precision: int = L.FP64,
             # setups of everything that has arrived are queued on the pool
             now = time.perf_counter() - t0
             while nxt < len(order) and order[nxt].arrival <= now:
-                futs[order[nxt].rid] = pool.submit(_setup, order[nxt], precision, t0)
+                futs[order[nxt].rid] = pool.submit(_setup, order[nxt], precision, t0, cache)
                 nxt += 1
             if spec.rid not in futs:
-                futs[spec.rid] = pool.submit(_setup, spec, precision, t0)
+                futs[spec.rid] = pool.submit(_setup, spec, precision, t0, cache)
                 nxt = max(nxt, order.index(spec) + 1)
             job = futs.pop(spec.rid).result()
             s = L.System(job.pattern, precision)
@@ -253,7 +287,7 @@ def serve_sequential(specs: List[W.RequestSpec], precision: int = L.FP64,
             job.infos = res["info"]
             job.t_done = time.perf_counter() - t0
             s.close()
-            job.pattern.close()
+            job.pattern = None
             job.system = None
             done.append(job)
     finally:

@@ -191,20 +191,26 @@ def request_stream(count: int, rate: float = 200.0, *, seed: int = 0,
     return out
 
 
-def request_scenario(spec: RequestSpec) -> Scenario:
+def request_mesh(kind: str, side: int) -> Mesh:
+    """The structured mesh of a request (depends on its kind and size only)."""
+    return quad_mesh(side, side, 1000.0, 1000.0) if kind == "quad" else hex_mesh(side, side, side)
+
+
+def request_scenario(spec: RequestSpec, mesh: Optional[Mesh] = None) -> Scenario:
     """The flow problem of a request: Dirichlet heads on a random pair of
     opposite faces (random values), one sink at a random interior node,
     implicit Euler from h = 0 (2-D: 1000 m square, 50 m correlation, dt
-    3600 s; 3-D: unit cells, 8-cell correlation, dt 100 s)."""
+    3600 s; 3-D: unit cells, 8-cell correlation, dt 100 s).  `mesh` may be a
+    shared request_mesh(kind, side)."""
     rng = np.random.Generator(np.random.PCG64(spec.seed))
     n = spec.side
     if spec.kind == "quad":
-        m = quad_mesh(n, n, 1000.0, 1000.0)
+        m = mesh if mesh is not None else request_mesh("quad", n)
         K = lognormal_conductivity(m, np.log(1e-4), spec.sigma_ln_k, 50.0, spec.seed)
         length, dt, q = 1000.0, 3600.0, -0.01
         axes = 2
     else:
-        m = hex_mesh(n, n, n)
+        m = mesh if mesh is not None else request_mesh("hex", n)
         K = lognormal_conductivity(m, np.log(1e-4), spec.sigma_ln_k, 8.0, spec.seed)
         length, dt, q = float(n), 100.0, -1e-4
         axes = 3
