@@ -14,6 +14,7 @@
  *   IncorporateBoundaryConditions -> SetKnownX_i         ogsx_set_dirichlet
  *   Linear_EQS::Solver                                   ogsx_solve
  *   Linear_EQS::Write_BIN                                ogsx_write_bin
+ *   CalIntegrationPointValue -> Cal_Velocity             ogsx_gauss_velocity
  *
  * Conventions: every function returns int status (0 = OK, < 0 = error; the
  * message is in ogsx_last_error()); no exception crosses the ABI; pointers
@@ -155,6 +156,23 @@ int ogsx_system_initialize(ogsx_system* s);
  * tim_type STEADY, fem_ele_std.cpp:8543; for LIQUID the same operator as a
  * zero storage coefficient).  u_old is the system's OGSX_VEC_UOLD vector. */
 int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady);
+
+/* Darcy velocity at the Gauss points of every element of a LIQUID_FLOW
+ * system from its current solution (OGSX_VEC_X, the new time level):
+ * CRFProcess::CalIntegrationPointValue (rf_pcs.cpp:5915-5994) ->
+ * CFiniteElementStd::Cal_Velocity (fem_ele_std.cpp:6554-6868),
+ * v = -(K/mu)(grad p + rho_g e_dim) per Gauss point, in the layout of
+ * ElementValue::Velocity: [n_elem][3][ngp], components >= dim zero.
+ *   heat != NULL  writes it into heat's Gauss-point velocity on the device
+ *                 (as if passed in ogsx_elements.vel), switching advection on
+ *                 for heat's next ogsx_assemble; heat must be a HEAT_TRANSPORT
+ *                 system on the same mesh (element e of both is the same
+ *                 element) with the same n_gauss.  Ordered after flow's
+ *                 pending work; flow's later work waits for the copy.
+ *   host_out      if not NULL, also receives the n_elem*3*ngp values
+ *                 (blocking).
+ * Neither system may have a solve in progress. */
+int ogsx_gauss_velocity(ogsx_system* flow, ogsx_system* heat, double* host_out);
 
 /* IncorporateSourceTerms (rf_pcs.cpp:8002; point sources :8447):
  * b[idx[k]] += val[k] in list order. */

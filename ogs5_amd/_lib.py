@@ -72,6 +72,7 @@ SIGNATURES = {
     "ogsx_system_set_elements": (C.c_int, [_P, C.POINTER(Elements)]),
     "ogsx_system_initialize": (C.c_int, [_P]),
     "ogsx_assemble": (C.c_int, [_P, C.c_double, C.c_double, C.c_int]),
+    "ogsx_gauss_velocity": (C.c_int, [_P, _P, _P]),
     "ogsx_add_rhs": (C.c_int, [_P, _I64, _P, _P]),
     "ogsx_set_dirichlet": (C.c_int, [_P, _I64, _P, _P, C.c_int]),
     "ogsx_solve": (C.c_int, [_P, C.POINTER(SolverConfig), C.POINTER(SolveInfo)]),
@@ -229,6 +230,7 @@ class System:
         self.nnz = pattern.nnz
         self.precision = precision
         self._keep = []
+        self.n_gauss = 3
 
     def set_elements(self, *, process, dim, x, y=None, z=None, mass_coef, cond, cond_stride=1,
                      n_gauss=3, tuf=1.0, mu=1.0, geo_area=1.0, adv_coef=0.0, vel=None,
@@ -241,6 +243,7 @@ class System:
                      tuf, mu, geo_area, adv_coef,
                      None if arrs[5] is None else _ptr(arrs[5]).value, rho_g)
         check(lib().ogsx_system_set_elements(self.h, C.byref(e)), "ogsx_system_set_elements")
+        self.n_gauss = int(n_gauss)
 
     def initialize(self):
         check(lib().ogsx_system_initialize(self.h), "ogsx_system_initialize")
@@ -248,6 +251,21 @@ class System:
     def assemble(self, dt, theta, steady=False):
         check(lib().ogsx_assemble(self.h, float(dt), float(theta), int(bool(steady))),
               "ogsx_assemble")
+
+    def gauss_velocity(self, heat: "System" = None, host: bool = True):
+        """Darcy velocity at the Gauss points from this LIQUID_FLOW system's
+        solution (ogsx_gauss_velocity).  heat: a HEAT_TRANSPORT system on the
+        same mesh that takes it as its advection velocity on the device.
+        Returns the [n_elem, 3, ngp] array when host is true."""
+        out = None
+        if host:
+            et, ng = self.pattern.etype, self.n_gauss
+            ngp = {LINE2: ng, QUAD4: ng * ng, HEX8: ng ** 3, TET4: 5}[et]
+            out = np.empty((self.pattern.n_elem, 3, ngp))
+        check(lib().ogsx_gauss_velocity(self.h, heat.h if heat is not None else None,
+                                        _ptr(out) if out is not None else None),
+              "ogsx_gauss_velocity")
+        return out
 
     def add_rhs(self, idx, val):
         idx, val = _i64(idx), _f64(val)

@@ -163,7 +163,7 @@ struct ogsx_system {
     SolveCtx sc;
     MemberSet* solo = nullptr;         // set of one for ogsx_solve
     ogsx_group* group = nullptr;       // group the system belongs to, if any
-    hipEvent_t ready = nullptr;        // marks the system's stream for the group stream
+    hipEvent_t ready = nullptr;        // marks the system's stream for another stream
     // BC / ST
     int8_t* bcflag = nullptr;
     double* bcg = nullptr;
@@ -181,6 +181,8 @@ struct ogsx_system {
     double tuf = 1, mu = 1, geo_area = 1, adv_coef = 0, rho_g = 0;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr, *mass_coef = nullptr, *cond = nullptr,
            *vel = nullptr, *tabN = nullptr, *tabdN = nullptr, *tabW = nullptr;
+    double* gvel = nullptr;  // Gauss-point velocity computed for the host only
+    size_t gvel_cap = 0;
     // persistent Linear_EQS::bNorm (equation_class.cpp:135)
     double bnorm = 1.0;
     // profiling
@@ -914,6 +916,25 @@ void launch_assemble_et(ogsx_system* s, const AsmArgs& a)
     }
 }
 
+template <int ET>
+void launch_velocity_et(ogsx_system* s, const VelArgs& a)
+{
+    const int grid = (a.n_elem + 255) / 256;
+    if (s->cond_stride == 1)
+        k_gauss_velocity<ET, true><<<grid, 256, 0, s->stream>>>(a);
+    else
+        k_gauss_velocity<ET, false><<<grid, 256, 0, s->stream>>>(a);
+}
+
+// Makes `waiter`'s stream wait for the work queued so far on `s`'s stream.
+int order_after(ogsx_system* s, ogsx_system* waiter)
+{
+    if (!s->ready) HIP_TRY(hipEventCreateWithFlags(&s->ready, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(s->ready, s->stream));
+    HIP_TRY(hipStreamWaitEvent(waiter->stream, s->ready, 0));
+    return OGSX_OK;
+}
+
 // Copies host lists to device buffers that grow on demand.
 template <typename T>
 int upload_list(ogsx_system* s, T** dptr, int* cap, const T* src, int count)
@@ -997,6 +1018,76 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
     prof_end(s, pk);
     s->a_version++;
     HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+int ogsx_gauss_velocity(ogsx_system* flow, ogsx_system* heat, double* host_out)
+{
+    if (!flow) return fail(OGSX_EINVAL, "null flow system");
+    if (!heat && !host_out) return fail(OGSX_EINVAL, "heat and host_out are both NULL");
+    if (heat == flow) return fail(OGSX_EINVAL, "heat and flow are the same system");
+    if (flow->sc.active || (heat && heat->sc.active))
+        return fail(OGSX_ESTATE, "system has a solve in progress");
+    if (!flow->elements_set || flow->process != OGSX_LIQUID_FLOW)
+        return fail(OGSX_EINVAL, "flow must be a LIQUID_FLOW system with its elements set");
+    const PatternHost& h = *flow->pat;
+    const size_t need = (size_t)h.n_elem * 3 * flow->ngp;
+    double* dst = nullptr;
+    if (heat) {
+        if (!heat->elements_set || heat->process != OGSX_HEAT_TRANSPORT)
+            return fail(OGSX_EINVAL, "heat must be a HEAT_TRANSPORT system with its elements set");
+        if (heat->pat->n_elem != h.n_elem || heat->pat->etype != h.etype || heat->ngp != flow->ngp)
+            return fail(OGSX_EINVAL, "heat is not on the flow mesh with the same n_gauss");
+        if (need > heat->vel_cap) {
+            OK_TRY(dalloc(heat, &heat->vel, need));
+            heat->vel_cap = need;
+        }
+        dst = heat->vel;
+        // heat's queued work (an assembly reading the old velocity, the
+        // allocation) comes first
+        OK_TRY(order_after(heat, flow));
+    } else {
+        if (need > flow->gvel_cap) {
+            OK_TRY(dalloc(flow, &flow->gvel, need));
+            flow->gvel_cap = need;
+        }
+        dst = flow->gvel;
+    }
+    if (h.n_elem > 0) {
+        VelArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.conn = flow->conn;
+        a.x = flow->ex;
+        a.y = flow->ey;
+        a.z = flow->ez;
+        a.cond = flow->cond;
+        a.tabdN = flow->tabdN;
+        a.u = flow->v64.x;
+        a.vel = dst;
+        a.n_elem = (int)h.n_elem;
+        a.cond_stride = flow->cond_stride;
+        a.ngp = flow->ngp;
+        a.tuf = flow->tuf;
+        a.mu = flow->mu;
+        a.rho_g = flow->rho_g;
+        switch (h.etype) {
+            case OGSX_LINE2: launch_velocity_et<OGSX_LINE2>(flow, a); break;
+            case OGSX_QUAD4: launch_velocity_et<OGSX_QUAD4>(flow, a); break;
+            case OGSX_HEX8: launch_velocity_et<OGSX_HEX8>(flow, a); break;
+            case OGSX_TET4: launch_velocity_et<OGSX_TET4>(flow, a); break;
+            default: return fail(OGSX_EINVAL, "unsupported element type");
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    if (heat) {
+        OK_TRY(order_after(flow, heat));
+        heat->use_vel = true;
+    }
+    if (host_out) {
+        if (need > 0)
+            HIP_TRY(hipMemcpyAsync(host_out, dst, 8 * need, hipMemcpyDeviceToHost, flow->stream));
+        HIP_TRY(hipStreamSynchronize(flow->stream));
+    }
     return OGSX_OK;
 }
 

@@ -483,6 +483,82 @@ template <> struct ETraits<OGSX_QUAD4> { static constexpr int NPE = 4, D = 2, NW
 template <> struct ETraits<OGSX_HEX8> { static constexpr int NPE = 8, D = 3, NW = 3; };
 template <> struct ETraits<OGSX_TET4> { static constexpr int NPE = 4, D = 3, NW = 1; };
 
+// Jacobian determinant and global shape-function gradients at one Gauss
+// point: CElement::computeJacobian (FEM/fem_ele.cpp:585-729) and
+// ComputeGradShapefct (:1128-1160).  dNg are the local gradients [D][NPE];
+// dshp receives the global ones [D][NPE].
+template <int ET>
+__device__ __forceinline__ double elem_gradients(const double* X, const double* Y,
+                                                 const double* Z, const double* dNg,
+                                                 double* dshp)
+{
+    constexpr int NPE = ETraits<ET>::NPE, D = ETraits<ET>::D;
+    double J[D * D];
+#pragma unroll
+    for (int i = 0; i < D * D; i++) J[i] = 0.0;
+    double inv[D * D], det;
+    if constexpr (D == 1) {
+        const double dx = X[1] - X[0];
+        J[0] = 0.5 * dx;
+        inv[0] = 2.0 / dx;
+        det = J[0];
+    } else if constexpr (D == 2) {
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+            J[0] += X[i] * dNg[i];
+            J[1] += Y[i] * dNg[i];
+            J[2] += X[i] * dNg[NPE + i];
+            J[3] += Y[i] * dNg[NPE + i];
+        }
+        det = J[0] * J[3] - J[1] * J[2];
+        inv[0] = J[3];
+        inv[1] = -J[1];
+        inv[2] = -J[2];
+        inv[3] = J[0];
+#pragma unroll
+        for (int i = 0; i < 4; i++) inv[i] /= det;
+    } else {
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+            const double d0 = dNg[i], d1 = dNg[NPE + i], d2 = dNg[2 * NPE + i];
+            J[0] += X[i] * d0;
+            J[1] += Y[i] * d0;
+            J[2] += Z[i] * d0;
+            J[3] += X[i] * d1;
+            J[4] += Y[i] * d1;
+            J[5] += Z[i] * d1;
+            J[6] += X[i] * d2;
+            J[7] += Y[i] * d2;
+            J[8] += Z[i] * d2;
+        }
+        det = J[0] * (J[4] * J[8] - J[7] * J[5]) + J[6] * (J[1] * J[5] - J[4] * J[2]) +
+              J[3] * (J[2] * J[7] - J[8] * J[1]);
+        inv[0] = J[4] * J[8] - J[7] * J[5];
+        inv[1] = J[2] * J[7] - J[1] * J[8];
+        inv[2] = J[1] * J[5] - J[2] * J[4];
+        inv[3] = J[5] * J[6] - J[8] * J[3];
+        inv[4] = J[0] * J[8] - J[6] * J[2];
+        inv[5] = J[2] * J[3] - J[5] * J[0];
+        inv[6] = J[3] * J[7] - J[6] * J[4];
+        inv[7] = J[1] * J[6] - J[7] * J[0];
+        inv[8] = J[0] * J[4] - J[3] * J[1];
+#pragma unroll
+        for (int i = 0; i < 9; i++) inv[i] /= det;
+    }
+    // global gradients dshp[k][i] = sum_m inv[k][m] dN_local[m][i]
+#pragma unroll
+    for (int i = 0; i < NPE; i++) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; m++) s += inv[k * D + m] * dNg[m * NPE + i];
+            dshp[k * NPE + i] = s;
+        }
+    }
+    return det;
+}
+
 // Local matrices follow CElement::computeJacobian / ComputeGradShapefct
 // (FEM/fem_ele.cpp:585-729, 1128-1160), CalcMass (FEM/fem_ele_std.cpp:3753),
 // CalcLaplace (:4948), CalcAdvection (:5280), Assemble_Gravity (:5859);
@@ -541,70 +617,8 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
     for (int gp = 0; gp < a.ngp; gp++) {
         const double* Ng = a.tabN + gp * NPE;
         const double* dNg = a.tabdN + gp * D * NPE;
-        double J[D * D];
-#pragma unroll
-        for (int i = 0; i < D * D; i++) J[i] = 0.0;
-        double inv[D * D], det;
-        if constexpr (D == 1) {
-            const double dx = X[1] - X[0];
-            J[0] = 0.5 * dx;
-            inv[0] = 2.0 / dx;
-            det = J[0];
-        } else if constexpr (D == 2) {
-#pragma unroll
-            for (int i = 0; i < NPE; i++) {
-                J[0] += X[i] * dNg[i];
-                J[1] += Y[i] * dNg[i];
-                J[2] += X[i] * dNg[NPE + i];
-                J[3] += Y[i] * dNg[NPE + i];
-            }
-            det = J[0] * J[3] - J[1] * J[2];
-            inv[0] = J[3];
-            inv[1] = -J[1];
-            inv[2] = -J[2];
-            inv[3] = J[0];
-#pragma unroll
-            for (int i = 0; i < 4; i++) inv[i] /= det;
-        } else {
-#pragma unroll
-            for (int i = 0; i < NPE; i++) {
-                const double d0 = dNg[i], d1 = dNg[NPE + i], d2 = dNg[2 * NPE + i];
-                J[0] += X[i] * d0;
-                J[1] += Y[i] * d0;
-                J[2] += Z[i] * d0;
-                J[3] += X[i] * d1;
-                J[4] += Y[i] * d1;
-                J[5] += Z[i] * d1;
-                J[6] += X[i] * d2;
-                J[7] += Y[i] * d2;
-                J[8] += Z[i] * d2;
-            }
-            det = J[0] * (J[4] * J[8] - J[7] * J[5]) + J[6] * (J[1] * J[5] - J[4] * J[2]) +
-                  J[3] * (J[2] * J[7] - J[8] * J[1]);
-            inv[0] = J[4] * J[8] - J[7] * J[5];
-            inv[1] = J[2] * J[7] - J[1] * J[8];
-            inv[2] = J[1] * J[5] - J[2] * J[4];
-            inv[3] = J[5] * J[6] - J[8] * J[3];
-            inv[4] = J[0] * J[8] - J[6] * J[2];
-            inv[5] = J[2] * J[3] - J[5] * J[0];
-            inv[6] = J[3] * J[7] - J[6] * J[4];
-            inv[7] = J[1] * J[6] - J[7] * J[0];
-            inv[8] = J[0] * J[4] - J[3] * J[1];
-#pragma unroll
-            for (int i = 0; i < 9; i++) inv[i] /= det;
-        }
-        // global gradients dshp[k][i] = sum_m inv[k][m] dN_local[m][i]
         double dshp[D * NPE];
-#pragma unroll
-        for (int i = 0; i < NPE; i++) {
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                double s = 0.0;
-#pragma unroll
-                for (int m = 0; m < D; m++) s += inv[k * D + m] * dNg[m * NPE + i];
-                dshp[k * NPE + i] = s;
-            }
-        }
+        const double det = elem_gradients<ET>(X, Y, Z, dNg, dshp);
         double fkt = fabs(det);
         if (D != 3) fkt *= a.geo_area;
         fkt = fkt * a.tabW[gp * 3 + 0];
@@ -709,6 +723,85 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
         if (do_grav) a.b[nd[i]] += G[i];
 #pragma unroll
         for (int j = 0; j < NPE; j++) a.A[sl[i * NPE + j]] += lhs[i * NPE + j];
+    }
+}
+
+// ------------------------------------------------------------------------
+// Gauss-point Darcy velocity of a LIQUID_FLOW system (one thread per element)
+// ------------------------------------------------------------------------
+struct VelArgs {
+    const int32_t* conn;
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* cond;
+    const double* tabdN;  // [ngp][edim][npe]
+    const double* u;      // nodal pressure (head) at the new time level
+    double* vel;          // out: [E][3][ngp]
+    int n_elem;
+    int cond_stride;
+    int ngp;
+    int pad;
+    double tuf, mu, rho_g;
+};
+
+// CFiniteElementStd::Cal_Velocity (FEM/fem_ele_std.cpp:6554-6868) for a
+// single-phase flow: at each Gauss point g = grad N . u (:6655-6661), plus
+// rho*g on the last component when gravity is on (:6748), then
+// v_i = -sum_j mat_ij g_j / tuf with mat = tuf*K/mu from CalCoefLaplace
+// (:2298-2301), components >= dim left zero.  ISO skips the zero
+// off-diagonal products, which subtract exact zeros.
+template <int ET, bool ISO>
+__global__ __launch_bounds__(256) void k_gauss_velocity(VelArgs a)
+{
+    constexpr int NPE = ETraits<ET>::NPE, D = ETraits<ET>::D;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.n_elem) return;
+    double X[NPE], Y[NPE], Z[NPE], nv[NPE];
+#pragma unroll
+    for (int i = 0; i < NPE; i++) {
+        const int nd = a.conn[(int64_t)e * NPE + i];
+        X[i] = a.x[nd];
+        Y[i] = (D >= 2) ? a.y[nd] : 0.0;
+        Z[i] = (D >= 3) ? a.z[nd] : 0.0;
+        nv[i] = a.u[nd];
+    }
+    double mat[D * D];
+#pragma unroll
+    for (int i = 0; i < D * D; i++) mat[i] = 0.0;
+    if (ISO) {
+#pragma unroll
+        for (int i = 0; i < D; i++) mat[i * D + i] = a.tuf * a.cond[e] / a.mu;
+    } else {
+#pragma unroll
+        for (int i = 0; i < D * D; i++)
+            mat[i] = a.tuf * a.cond[(int64_t)e * a.cond_stride + i] / a.mu;
+    }
+    double* out = a.vel + (int64_t)e * 3 * a.ngp;
+    for (int gp = 0; gp < a.ngp; gp++) {
+        double dshp[D * NPE];
+        elem_gradients<ET>(X, Y, Z, a.tabdN + gp * D * NPE, dshp);
+        double g[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NPE; j++) s += nv[j] * dshp[i * NPE + j];
+            g[i] = s;
+        }
+        if (a.rho_g != 0.0) g[D - 1] += a.rho_g;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            double v = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                if (ISO && j != i) continue;
+                v -= mat[D * i + j] * g[j] / a.tuf;
+            }
+            out[i * a.ngp + gp] = v;
+        }
+#pragma unroll
+        for (int i = D; i < 3; i++) out[i * a.ngp + gp] = 0.0;
     }
 }
 

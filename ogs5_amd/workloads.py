@@ -147,6 +147,38 @@ def tet_heat(n: int = 96, *, seed: int = 3, conductivity: float = 2.0, rho_c: fl
                     meta=dict(seed=seed, peclet_min=float(pe.min()), peclet_max=float(pe.max())))
 
 
+def coupled_flow_heat(n: int = 32, *, seed: int = 5, mean_ln_k: float = np.log(1e-12),
+                      sigma_ln_k: float = 1.0, corr_cells: float = 6.0, mu: float = 1.0e-3,
+                      storage: float = 1.0e-7, rho_g: float = 1000.0 * 9.81, dp: float = 2.0e5,
+                      conductivity: float = 2.0, rho_c: float = 2.5e6,
+                      fluid_rho_c: float = 4.2e6, t0: float = 10.0, t_inlet: float = 60.0,
+                      dt: float = 2.0e4, n_steps: int = 10, theta: float = 0.5):
+    """Pressure-driven flow through a heterogeneous block that carries heat:
+    n^3 cubes split into 6 tets each; LIQUID_FLOW in pressure form with
+    gravity (K/mu, rho*g on z) and hydrostatic faces p = dp - rho_g z at
+    x = 0 and p = -rho_g z at x = L, starting from p = 0 (implicit Euler);
+    HEAT_TRANSPORT advected by the Gauss-point Darcy velocity of each step's
+    new pressure, inlet face x = 0 held at t_inlet (Crank-Nicolson).
+    Returns (flow, heat); heat.vel is None, the velocity comes from flow."""
+    m = tet_mesh(n, n, n)
+    lx = float(m.x.max())
+    K = lognormal_conductivity(m, mean_ln_k, sigma_ln_k, corr_cells, seed)
+    left = m.nodes_where(0, 0.0)
+    right = m.nodes_where(0, lx)
+    bc_idx = np.concatenate([left, right]).astype(np.int64)
+    bc_val = np.concatenate([dp - rho_g * m.z[left], -rho_g * m.z[right]])
+    flow = Scenario(f"coupled_flow_{n}^3x6", m, L.LIQUID_FLOW, np.full(m.n_elem, storage), K, 1,
+                    bc_idx, bc_val, np.zeros(0, np.int64), np.zeros(0), np.zeros(m.n_nodes),
+                    [Step(dt, 1.0)] * n_steps, mu=mu, rho_g=rho_g,
+                    meta=dict(seed=seed, sigma_ln_k=sigma_ln_k))
+    heat = Scenario(f"coupled_heat_{n}^3x6", m, L.HEAT_TRANSPORT, np.full(m.n_elem, rho_c),
+                    np.full(m.n_elem, conductivity), 1, left.astype(np.int64),
+                    np.full(left.size, t_inlet), np.zeros(0, np.int64), np.zeros(0),
+                    np.full(m.n_nodes, t0), [Step(dt, theta)] * n_steps,
+                    adv_coef=fluid_rho_c)
+    return flow, heat
+
+
 def ensemble(count: int = 256, *, side_range=(64, 512), sigma_range=(0.5, 2.5),
              n_steps: int = 10, seed: int = 0) -> List[Scenario]:
     """Independent 2-D flow instances with the quad_flow set-up (1000 m
@@ -274,6 +306,40 @@ def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SY
     if keep_steps:
         out["steps"] = xs
     return out
+
+
+def run_coupled_gpu(flow: Scenario, heat: Scenario, *, precision: int = L.FP64,
+                    tol: Optional[float] = None, keep_vel: bool = False) -> dict:
+    """Each time step: the flow step (CG, symmetric Dirichlet), the
+    Gauss-point velocity of its new pressure written into the heat system on
+    the device (ogsx_gauss_velocity), then the heat step (BiCGStab, row-only
+    Dirichlet: advection makes it non-symmetric).  Both systems share the
+    mesh's CRS pattern."""
+    pat = L.Pattern(flow.n, flow.mesh.etype, flow.mesh.conn)
+    sf = L.System(pat, precision)
+    sf.set_elements(**flow.element_kwargs())
+    sh = L.System(pat, precision)
+    sh.set_elements(**heat.element_kwargs())
+    for sc, s in ((flow, sf), (heat, sh)):
+        s.set_vector(L.VEC_UOLD, sc.u0)
+        s.set_vector(L.VEC_X, sc.u0)
+    infos_f, infos_h, vel = [], [], None
+    try:
+        for k in range(len(flow.steps)):
+            for sc, s, infos, method, dirichlet in (
+                    (flow, sf, infos_f, L.CG, L.DIRICHLET_SYMMETRIC),
+                    (heat, sh, infos_h, L.BICGSTAB, L.DIRICHLET_ROW)):
+                _prepare_step(sc, s, k, dirichlet)
+                infos.append(s.solve(method=method, tol=sc.tol if tol is None else tol,
+                                     max_iter=sc.max_iter))
+                s.copy_vector(L.VEC_UOLD, L.VEC_X)
+                if s is sf:
+                    vel = sf.gauss_velocity(sh, host=keep_vel)
+        return dict(x_flow=sf.get_vector(L.VEC_X), x_heat=sh.get_vector(L.VEC_X),
+                    info_flow=infos_f, info_heat=infos_h, vel=vel)
+    finally:
+        sh.close()
+        sf.close()
 
 
 def _prepare_step(sc: Scenario, s, k: int, dirichlet: int):

@@ -60,6 +60,7 @@ def lib():
             "orc_pattern_fill": (None, [C.c_int64, C.c_int64, C.c_int, P, P, P, P]),
             "orc_element_matrices": (None, [C.POINTER(Problem), C.c_int64, P, P, P, P]),
             "orc_element_system": (None, [C.POINTER(Problem), C.c_int64, P, P]),
+            "orc_gauss_velocity": (None, [C.POINTER(Problem), P, P]),
             "orc_eqs_init": (None, [C.POINTER(Eqs), C.c_int64, C.c_int64, P, P, P, P, P, P]),
             "orc_eqs_config": (None, [C.POINTER(Eqs), C.c_int, C.c_int, C.c_double, C.c_int]),
             "orc_eqs_initialize": (None, [C.POINTER(Eqs)]),
@@ -122,9 +123,9 @@ class OracleProblem:
 
     def set(self, **kw):
         for k, v in kw.items():
-            if k == "u_old":
-                self.arrays["u_old"] = np.ascontiguousarray(v, dtype=np.float64)
-                self.s.u_old = _p(self.arrays["u_old"])
+            if k in ("u_old", "vel"):
+                self.arrays[k] = np.ascontiguousarray(v, dtype=np.float64)
+                setattr(self.s, k, _p(self.arrays[k]))
             elif k == "steady":
                 self.s.steady = int(bool(v))
             else:
@@ -135,6 +136,14 @@ class OracleProblem:
         M, L, A, G = (np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n)), np.zeros(n))
         lib().orc_element_matrices(C.byref(self.s), e, _p(M), _p(L), _p(A), _p(G))
         return M, L, A, G
+
+    def gauss_velocity(self, u):
+        """[n_elem][3][ngp] Gauss-point Darcy velocity from nodal values u."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        ngp = lib().orc_ngp(self.s.etype, self.s.n_gauss)
+        vel = np.empty((self.s.n_elem, 3, ngp))
+        lib().orc_gauss_velocity(C.byref(self.s), _p(u), _p(vel))
+        return vel
 
     def element_system(self, e):
         n = self.npe

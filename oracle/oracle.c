@@ -579,6 +579,44 @@ void orc_element_matrices(const orc_problem* p, int64_t e, double* mass,
                 if (i > j) mass[j * npe + i] = mass[i * npe + j];
 }
 
+/* Cal_Velocity (fem_ele_std.cpp:6554-6868), single-phase branch:
+ *   NodalVal[i] = u(node i)                                    (:6600-6604)
+ *   per Gauss point (GetGaussData / getGradShapefunctValues order):
+ *     vel[i] = sum_j NodalVal[j] dshapefct[i][j]                (:6655-6661)
+ *     vel[dim-1] += gravity_constant*rho when gravity is on      (:6748)
+ *     Velocity(i,gp) = 0 - sum_j mat[dim*i+j] vel[j] / tuf       (:6805-6826)
+ *   with mat from CalCoefLaplace(true) = tuf*K/mu (:2298-2301); rows >= dim
+ *   keep the zero of tmp_gp_velocity (:6627-6628). */
+void orc_gauss_velocity(const orc_problem* p, const double* u, double* vel)
+{
+    const int et = p->etype, npe = orc_npe(et);
+    const int ngp = orc_ngp(et, p->n_gauss), dim = p->dim;
+    for (int64_t e = 0; e < p->n_elem; e++) {
+        double X[8], Y[8], Z[8], dshp[24], mat[9], nodal[8];
+        load_coords(p, e, X, Y, Z);
+        laplace_tensor(p, e, mat);
+        for (int a = 0; a < npe; a++) nodal[a] = u[p->conn[e * npe + a]];
+        double* out = vel + e * 3 * ngp;
+        for (int i = 0; i < 3 * ngp; i++) out[i] = 0.0;
+        for (int gp = 0; gp < ngp; gp++) {
+            double fkt, g[3];
+            if (element_geometry(p, e, gp, X, Y, Z, dshp, &fkt) != 0) {
+                fprintf(stderr, "oracle: Jacobian determinant == 0 in element %lld\n",
+                        (long long)e);
+                abort();
+            }
+            for (int i = 0; i < dim; i++) {
+                g[i] = 0.0;
+                for (int j = 0; j < npe; j++) g[i] += nodal[j] * dshp[i * npe + j];
+            }
+            if (p->rho_g != 0.0) g[dim - 1] += p->rho_g;
+            for (int i = 0; i < dim; i++)
+                for (int j = 0; j < dim; j++)
+                    out[i * ngp + gp] -= mat[dim * i + j] * g[j] / p->tuf;
+        }
+    }
+}
+
 void orc_dense_multi(const double* A, int nrows, int ncols, const double* vec,
                      double* vec_result, double fac)
 {

@@ -93,6 +93,13 @@ void orc_element_matrices(const orc_problem* p, int64_t e, double* mass,
 void orc_element_system(const orc_problem* p, int64_t e, double* lhs,
                         double* rhs);
 
+/* Gauss-point Darcy velocity of a LIQUID_FLOW problem from the nodal values
+ * u at the new time level (CFiniteElementStd::Cal_Velocity,
+ * fem_ele_std.cpp:6554-6868, for every element as
+ * CRFProcess::CalIntegrationPointValue rf_pcs.cpp:5966-5993 does):
+ * vel[(e*3+i)*ngp+gp] = ElementValue::Velocity(i, gp). */
+void orc_gauss_velocity(const orc_problem* p, const double* u, double* vel);
+
 /* ---- Linear_EQS (FEM/equation_class.h:56) ------------------------------- */
 typedef struct {
     int64_t n, nnz;

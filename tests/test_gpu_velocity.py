@@ -1,0 +1,176 @@
+"""Gauss-point Darcy velocity on the device (ogsx_gauss_velocity) against
+the oracle's Cal_Velocity restatement, and the coupled LIQUID_FLOW ->
+HEAT_TRANSPORT time step that uses it.
+
+Bars: velocities within 1e-12 of the oracle's, relative to max|v|; the heat
+system assembled with the device velocity within 1e-12 (row-scaled) of the
+oracle's assembled with the oracle's velocity; coupled solutions within
+1e-10 relative L2 of a tight-tolerance oracle chain.
+"""
+import numpy as np
+import pytest
+
+import ogs5_amd as ox
+from ogs5_amd import mesh as M
+from ogs5_amd import workloads as W
+from oracle import driver
+
+pytestmark = pytest.mark.gpu
+
+VEL_TOL = 1e-12
+ENTRY_TOL = 1e-12
+SOL_TOL = 1e-10
+
+
+def _mesh(kind, rng):
+    if kind == "line":
+        m = M.line_mesh(40, 12.0)
+        m.x = m.x + np.r_[0.0, 0.05 * rng.uniform(-1, 1, m.n_nodes - 2), 0.0]
+    elif kind == "quad":
+        m = M.quad_mesh(14, 11, 7.0, 5.5)
+        m.x = m.x + 0.1 * rng.uniform(-1, 1, m.n_nodes)
+        m.y = m.y + 0.1 * rng.uniform(-1, 1, m.n_nodes)
+    elif kind == "hex":
+        m = M.hex_mesh(6, 5, 4)
+        for a in ("x", "y", "z"):
+            setattr(m, a, getattr(m, a) + 0.15 * rng.uniform(-1, 1, m.n_nodes))
+    else:
+        m = M.tet_mesh(5, 4, 4)
+    return m
+
+
+def _flow_pair(m, rng, *, tensor, n_gauss, rho_g, tuf=1.0, mu=1.3e-3):
+    """GPU LIQUID_FLOW system and the oracle problem on the same data, with a
+    random nodal pressure in the system's x vector."""
+    from oracle import ffi
+    D = m.dim
+    k = np.exp(rng.normal(np.log(1e-12), 1.0, m.n_elem))
+    if tensor:
+        T = np.eye(D) + 0.3 * rng.uniform(-1, 1, (D, D))
+        cond, stride = (k[:, None] * T.ravel()[None, :]).ravel(), D * D
+    else:
+        cond, stride = k, 1
+    u = 1e5 * rng.standard_normal(m.n_nodes)
+    kw = dict(process=ox.LIQUID_FLOW, mass_coef=np.full(m.n_elem, 1e-7), cond=cond,
+              cond_stride=stride, n_gauss=n_gauss, tuf=tuf, mu=mu, rho_g=rho_g)
+    pat = ox.Pattern(m.n_nodes, m.etype, m.conn)
+    s = ox.System(pat, ox.FP64)
+    s.set_elements(dim=D, x=m.x, y=m.y if D >= 2 else None, z=m.z if D >= 3 else None, **kw)
+    s.set_vector(ox.VEC_X, u)
+    prob = ffi.OracleProblem(m, **kw)
+    return pat, s, prob, u
+
+
+@pytest.mark.parametrize("kind,tensor,n_gauss,gravity", [
+    ("line", False, 3, True),
+    ("quad", False, 3, False),
+    ("quad", True, 2, True),
+    ("hex", True, 3, True),
+    ("hex", False, 2, False),
+    ("tet", True, 3, True),
+    ("tet", False, 3, False),
+])
+def test_velocity_matches_oracle(kind, tensor, n_gauss, gravity):
+    rng = np.random.default_rng(21)
+    m = _mesh(kind, rng)
+    pat, s, prob, u = _flow_pair(m, rng, tensor=tensor, n_gauss=n_gauss,
+                                 rho_g=9.81e3 if gravity else 0.0, tuf=3600.0)
+    v = s.gauss_velocity()
+    vr = prob.gauss_velocity(u)
+    print(f"[velocity] {kind} tensor={tensor} n_gauss={n_gauss} grav={gravity} "
+          f"E={m.n_elem} shape={v.shape}", flush=True)
+    assert v.shape == vr.shape
+    assert np.abs(v - vr).max() <= VEL_TOL * np.abs(vr).max()
+    assert not v[:, m.dim:, :].any()
+    s.close()
+
+
+def _row_scaled_err(a, a_ref, ptr):
+    n = ptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    rowmax = np.zeros(n)
+    np.maximum.at(rowmax, rows, np.abs(a_ref))
+    return (np.abs(a - a_ref) / np.maximum(rowmax[rows], 1e-300)).max()
+
+
+def test_velocity_into_heat_assembly():
+    """The device velocity becomes the heat system's advection velocity:
+    the assembled heat system equals the oracle's with the oracle velocity."""
+    from oracle import ffi
+    flow, heat = W.coupled_flow_heat(5, n_steps=1)
+    m = flow.mesh
+    rng = np.random.default_rng(4)
+    u = 2e5 * (1.0 - m.x / m.x.max()) - flow.rho_g * m.z + 1e3 * rng.standard_normal(m.n_nodes)
+    pat = ox.Pattern(m.n_nodes, m.etype, m.conn)
+    sf = ox.System(pat, ox.FP64)
+    sf.set_elements(**flow.element_kwargs())
+    sh = ox.System(pat, ox.FP64)
+    sh.set_elements(**heat.element_kwargs())
+    sf.set_vector(ox.VEC_X, u)
+    st = heat.steps[0]
+    # without a velocity the heat operator is symmetric
+    sh.set_vector(ox.VEC_UOLD, heat.u0)
+    sh.initialize()
+    sh.assemble(st.dt, st.theta, st.steady)
+    A0 = sh.get_matrix()
+    v = sf.gauss_velocity(sh, host=True)
+    sh.initialize()
+    sh.assemble(st.dt, st.theta, st.steady)
+    A, b = sh.get_matrix(), sh.get_vector(ox.VEC_B)
+
+    pf = ffi.OracleProblem(m, process=flow.process, mass_coef=flow.mass_coef, cond=flow.cond,
+                           cond_stride=flow.cond_stride, mu=flow.mu, rho_g=flow.rho_g)
+    vr = pf.gauss_velocity(u)
+    assert np.abs(v - vr).max() <= VEL_TOL * np.abs(vr).max()
+    ptr, col, diag = ffi.pattern(heat.n, m.conn)
+    eq = ffi.OracleEqs(ptr, col, diag)
+    ph = ffi.OracleProblem(m, process=heat.process, mass_coef=heat.mass_coef, cond=heat.cond,
+                           adv_coef=heat.adv_coef, vel=vr, dt=st.dt, theta=st.theta,
+                           u_old=heat.u0)
+    eq.initialize()
+    eq.assemble(ph)
+    assert _row_scaled_err(A, eq.a, ptr) < ENTRY_TOL
+    assert np.abs(b - eq.b).max() <= ENTRY_TOL * np.abs(eq.b).max()
+    assert _row_scaled_err(A0, eq.a, ptr) > 1e-6  # advection is on
+    sh.close()
+    sf.close()
+
+
+@pytest.mark.parametrize("prec", [ox.FP64, ox.FP32_IR])
+def test_coupled_flow_heat_matches_oracle(prec):
+    from oracle import ffi
+    flow, heat = W.coupled_flow_heat(6, n_steps=3)
+    ref = driver.run_oracle_coupled(flow, heat, ffi, tol=1e-14)
+    assert all(i["error"] < 1e-13 for i in ref["info_flow"] + ref["info_heat"])
+    res = W.run_coupled_gpu(flow, heat, precision=prec, tol=1e-13, keep_vel=True)
+    print(f"[coupled] prec={prec} n={flow.n} flow iters="
+          f"{[i['iterations'] for i in res['info_flow']]} heat iters="
+          f"{[i['iterations'] for i in res['info_heat']]}", flush=True)
+    for key in ("x_flow", "x_heat"):
+        rel = np.linalg.norm(res[key] - ref[key]) / np.linalg.norm(ref[key])
+        assert rel < SOL_TOL, (key, rel)
+    vr = ref["vel"]
+    assert np.abs(res["vel"] - vr).max() <= 1e-8 * np.abs(vr).max()
+    assert all(i["converged"] == 1 for i in res["info_flow"] + res["info_heat"])
+
+
+def test_velocity_argument_checks():
+    flow, heat = W.coupled_flow_heat(3, n_steps=1)
+    m = flow.mesh
+    pat = ox.Pattern(m.n_nodes, m.etype, m.conn)
+    sf = ox.System(pat, ox.FP64)
+    sf.set_elements(**flow.element_kwargs())
+    sh = ox.System(pat, ox.FP64)
+    sh.set_elements(**heat.element_kwargs())
+    with pytest.raises(ox.OgsxError):
+        sf.gauss_velocity(sf)  # same system
+    with pytest.raises(ox.OgsxError):
+        sh.gauss_velocity()  # not a flow system
+    other = W.coupled_flow_heat(4, n_steps=1)[1]
+    po = ox.Pattern(other.n, other.mesh.etype, other.mesh.conn)
+    so = ox.System(po, ox.FP64)
+    so.set_elements(**other.element_kwargs())
+    with pytest.raises(ox.OgsxError):
+        sf.gauss_velocity(so)  # another mesh
+    for s in (so, sh, sf):
+        s.close()
