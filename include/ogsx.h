@@ -274,14 +274,17 @@ int ogsx_group_solve(ogsx_group* g, const ogsx_solver_config* cfg,
                      ogsx_solve_info* infos);
 
 /* ---- measurement ------------------------------------------------------ */
-/* Enables HIP-event timing of every SpMV launch that ogsx_solve issues (on
- * the system's stream) and returns the running totals. */
+/* Enables HIP-event timing of every SpMV launch that ogsx_solve issues, of
+ * ogsx_assemble and of ogsx_gauss_velocity (on the system's stream) and
+ * returns the running totals; bytes_per_launch is the kernel's algorithmic
+ * HBM traffic per launch (SpMV, velocity; 0 for the assembly). */
 int ogsx_profile_enable(ogsx_system* s, int on);
 int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
                       int64_t* launches, double* bytes_per_launch);
 #define OGSX_PROF_SPMV64 0  /* FP64 SpMV kernel                              */
 #define OGSX_PROF_SPMV32 1  /* FP32 SpMV kernel (IR inner solve)             */
 #define OGSX_PROF_ASSEMBLY 2
+#define OGSX_PROF_VELOCITY 3 /* ogsx_gauss_velocity kernel (on the flow system) */
 /* SpMV storage/launch variant: 0 = autotune (time variants 1, 2, 3, 5, 6, 7,
  * 10, 11, 12 on this matrix at first use and keep the fastest; all of them sum each row in
  * CRS order over rounded products, bit-identical to CSparseMatrix::multiVec,

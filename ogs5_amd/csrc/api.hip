@@ -190,7 +190,7 @@ struct ogsx_system {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;  // (kind, pool index of start)
     size_t ev_next = 0;
-    ProfAcc prof_acc[3];
+    ProfAcc prof_acc[4];
     std::vector<void*> allocs;
     int64_t bytes = 0;  // device bytes allocated
 };
@@ -916,14 +916,46 @@ void launch_assemble_et(ogsx_system* s, const AsmArgs& a)
     }
 }
 
+// Elements per block of the staged velocity kernel: a multiple of 64 up to
+// 256 whose 3*ngp velocities fit 32 KiB of LDS; 0 when even 64 elements do
+// not fit 64 KiB (hexahedra with 4^3 points), which then store directly.
+int velocity_block(int ngp)
+{
+    const int per_wave = 64 * 3 * ngp;  // doubles for 64 elements
+    if (per_wave * 8 > 65536) return 0;
+    return 64 * std::max(1, std::min(4, 4096 / per_wave));
+}
+
+template <int ET, bool ISO>
+void launch_velocity_iso(ogsx_system* s, const VelArgs& a)
+{
+    const int epb = velocity_block(a.ngp);
+    if (epb > 0) {
+        const size_t lds = (size_t)epb * 3 * a.ngp * sizeof(double);
+        k_gauss_velocity<ET, ISO, true><<<(a.n_elem + epb - 1) / epb, epb, lds, s->stream>>>(a);
+    } else {
+        k_gauss_velocity<ET, ISO, false><<<(a.n_elem + 255) / 256, 256, 0, s->stream>>>(a);
+    }
+}
+
 template <int ET>
 void launch_velocity_et(ogsx_system* s, const VelArgs& a)
 {
-    const int grid = (a.n_elem + 255) / 256;
     if (s->cond_stride == 1)
-        k_gauss_velocity<ET, true><<<grid, 256, 0, s->stream>>>(a);
+        launch_velocity_iso<ET, true>(s, a);
     else
-        k_gauss_velocity<ET, false><<<grid, 256, 0, s->stream>>>(a);
+        launch_velocity_iso<ET, false>(s, a);
+}
+
+// Algorithmic HBM bytes of one k_gauss_velocity launch: every node's
+// coordinates and pressure once, per element its connectivity and tensor,
+// and the 3 x ngp velocities written.
+double velocity_bytes(const ogsx_system* s)
+{
+    const PatternHost& h = *s->pat;
+    const int npe = npe_of(h.etype), ed = edim_of(h.etype);
+    return (double)h.n * 8.0 * (ed + 1) +
+           (double)h.n_elem * (4.0 * npe + 8.0 * s->cond_stride + 24.0 * s->ngp);
 }
 
 // Makes `waiter`'s stream wait for the work queued so far on `s`'s stream.
@@ -1070,6 +1102,7 @@ int ogsx_gauss_velocity(ogsx_system* flow, ogsx_system* heat, double* host_out)
         a.tuf = flow->tuf;
         a.mu = flow->mu;
         a.rho_g = flow->rho_g;
+        const int pk = prof_begin(flow, OGSX_PROF_VELOCITY);
         switch (h.etype) {
             case OGSX_LINE2: launch_velocity_et<OGSX_LINE2>(flow, a); break;
             case OGSX_QUAD4: launch_velocity_et<OGSX_QUAD4>(flow, a); break;
@@ -1077,6 +1110,7 @@ int ogsx_gauss_velocity(ogsx_system* flow, ogsx_system* heat, double* host_out)
             case OGSX_TET4: launch_velocity_et<OGSX_TET4>(flow, a); break;
             default: return fail(OGSX_EINVAL, "unsupported element type");
         }
+        prof_end(flow, pk);
         HIP_TRY(hipGetLastError());
     }
     if (heat) {
@@ -1439,7 +1473,7 @@ int ogsx_profile_enable(ogsx_system* s, int on)
 int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms, int64_t* launches,
                       double* bytes_per_launch)
 {
-    if (!s || kind < 0 || kind > 2) return fail(OGSX_EINVAL, "bad argument");
+    if (!s || kind < 0 || kind > 3) return fail(OGSX_EINVAL, "bad argument");
     OK_TRY(prof_flush(s));
     if (total_ms) *total_ms = s->prof_acc[kind].ms;
     if (launches) *launches = s->prof_acc[kind].launches;
@@ -1448,6 +1482,8 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms, int64_t* launc
             *bytes_per_launch = spmv_bytes(s, s->fmt64, 8, 8);
         else if (kind == OGSX_PROF_SPMV32)
             *bytes_per_launch = spmv_bytes(s, s->fmt32, 4, 4);
+        else if (kind == OGSX_PROF_VELOCITY)
+            *bytes_per_launch = velocity_bytes(s);
         else
             *bytes_per_launch = 0.0;
     }

@@ -750,58 +750,81 @@ struct VelArgs {
 // rho*g on the last component when gravity is on (:6748), then
 // v_i = -sum_j mat_ij g_j / tuf with mat = tuf*K/mu from CalCoefLaplace
 // (:2298-2301), components >= dim left zero.  ISO skips the zero
-// off-diagonal products, which subtract exact zeros.
-template <int ET, bool ISO>
+// off-diagonal products, which subtract exact zeros.  Linear tetrahedra
+// have the same local gradients at all Gauss points, so their velocity is
+// computed once and repeated (the same values the per-point loop gives).
+//
+// STAGED: the block's velocities ([elements of the block][3][ngp], one
+// contiguous range of the output) are gathered in LDS and written with
+// coalesced stores; per-thread stores of 3*ngp consecutive doubles leave
+// every wave store touching 64 partial cache lines.  Needs
+// blockDim.x * 3 * ngp doubles of dynamic LDS.
+template <int ET, bool ISO, bool STAGED>
 __global__ __launch_bounds__(256) void k_gauss_velocity(VelArgs a)
 {
     constexpr int NPE = ETraits<ET>::NPE, D = ETraits<ET>::D;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.n_elem) return;
-    double X[NPE], Y[NPE], Z[NPE], nv[NPE];
+    extern __shared__ double vstage[];
+    const int e0 = blockIdx.x * blockDim.x;
+    const int e = e0 + threadIdx.x;
+    const int ngp = a.ngp, per = 3 * ngp;
+    if (e < a.n_elem) {
+        double X[NPE], Y[NPE], Z[NPE], nv[NPE];
 #pragma unroll
-    for (int i = 0; i < NPE; i++) {
-        const int nd = a.conn[(int64_t)e * NPE + i];
-        X[i] = a.x[nd];
-        Y[i] = (D >= 2) ? a.y[nd] : 0.0;
-        Z[i] = (D >= 3) ? a.z[nd] : 0.0;
-        nv[i] = a.u[nd];
-    }
-    double mat[D * D];
-#pragma unroll
-    for (int i = 0; i < D * D; i++) mat[i] = 0.0;
-    if (ISO) {
-#pragma unroll
-        for (int i = 0; i < D; i++) mat[i * D + i] = a.tuf * a.cond[e] / a.mu;
-    } else {
-#pragma unroll
-        for (int i = 0; i < D * D; i++)
-            mat[i] = a.tuf * a.cond[(int64_t)e * a.cond_stride + i] / a.mu;
-    }
-    double* out = a.vel + (int64_t)e * 3 * a.ngp;
-    for (int gp = 0; gp < a.ngp; gp++) {
-        double dshp[D * NPE];
-        elem_gradients<ET>(X, Y, Z, a.tabdN + gp * D * NPE, dshp);
-        double g[D];
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-            double s = 0.0;
-#pragma unroll
-            for (int j = 0; j < NPE; j++) s += nv[j] * dshp[i * NPE + j];
-            g[i] = s;
+        for (int i = 0; i < NPE; i++) {
+            const int nd = a.conn[(int64_t)e * NPE + i];
+            X[i] = a.x[nd];
+            Y[i] = (D >= 2) ? a.y[nd] : 0.0;
+            Z[i] = (D >= 3) ? a.z[nd] : 0.0;
+            nv[i] = a.u[nd];
         }
-        if (a.rho_g != 0.0) g[D - 1] += a.rho_g;
+        double mat[D * D];
 #pragma unroll
-        for (int i = 0; i < D; i++) {
-            double v = 0.0;
+        for (int i = 0; i < D * D; i++) mat[i] = 0.0;
+        if (ISO) {
 #pragma unroll
-            for (int j = 0; j < D; j++) {
-                if (ISO && j != i) continue;
-                v -= mat[D * i + j] * g[j] / a.tuf;
+            for (int i = 0; i < D; i++) mat[i * D + i] = a.tuf * a.cond[e] / a.mu;
+        } else {
+#pragma unroll
+            for (int i = 0; i < D * D; i++)
+                mat[i] = a.tuf * a.cond[(int64_t)e * a.cond_stride + i] / a.mu;
+        }
+        double* out = STAGED ? vstage + threadIdx.x * per : a.vel + (int64_t)e * per;
+        const int n_eval = (ET == OGSX_TET4) ? 1 : ngp;
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int gp = 0; gp < ngp; gp++) {
+            if (gp < n_eval) {
+                double dshp[D * NPE];
+                elem_gradients<ET>(X, Y, Z, a.tabdN + gp * D * NPE, dshp);
+                double g[D];
+#pragma unroll
+                for (int i = 0; i < D; i++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NPE; j++) s += nv[j] * dshp[i * NPE + j];
+                    g[i] = s;
+                }
+                if (a.rho_g != 0.0) g[D - 1] += a.rho_g;
+#pragma unroll
+                for (int i = 0; i < D; i++) {
+                    double vi = 0.0;
+#pragma unroll
+                    for (int j = 0; j < D; j++) {
+                        if (ISO && j != i) continue;
+                        vi -= mat[D * i + j] * g[j] / a.tuf;
+                    }
+                    v[i] = vi;
+                }
             }
-            out[i * a.ngp + gp] = v;
-        }
 #pragma unroll
-        for (int i = D; i < 3; i++) out[i * a.ngp + gp] = 0.0;
+            for (int i = 0; i < 3; i++) out[i * ngp + gp] = v[i];
+        }
+    }
+    if constexpr (STAGED) {
+        __syncthreads();
+        const int cnt = min((int)blockDim.x, a.n_elem - e0);
+        const int total = cnt * per;
+        double* dst = a.vel + (int64_t)e0 * per;
+        for (int i = threadIdx.x; i < total; i += blockDim.x) dst[i] = vstage[i];
     }
 }
 

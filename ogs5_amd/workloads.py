@@ -309,17 +309,17 @@ def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SY
 
 
 def run_coupled_gpu(flow: Scenario, heat: Scenario, *, precision: int = L.FP64,
-                    tol: Optional[float] = None, keep_vel: bool = False) -> dict:
+                    tol: Optional[float] = None, keep_vel: bool = False,
+                    systems: Optional[tuple] = None) -> dict:
     """Each time step: the flow step (CG, symmetric Dirichlet), the
     Gauss-point velocity of its new pressure written into the heat system on
     the device (ogsx_gauss_velocity), then the heat step (BiCGStab, row-only
     Dirichlet: advection makes it non-symmetric).  Both systems share the
-    mesh's CRS pattern."""
-    pat = L.Pattern(flow.n, flow.mesh.etype, flow.mesh.conn)
-    sf = L.System(pat, precision)
-    sf.set_elements(**flow.element_kwargs())
-    sh = L.System(pat, precision)
-    sh.set_elements(**heat.element_kwargs())
+    mesh's CRS pattern.  systems: (pattern, flow system, heat system) made
+    by make_coupled_systems, kept open for reuse; otherwise made and closed
+    here."""
+    own = systems is None
+    pat, sf, sh = make_coupled_systems(flow, heat, precision) if own else systems
     for sc, s in ((flow, sf), (heat, sh)):
         s.set_vector(L.VEC_UOLD, sc.u0)
         s.set_vector(L.VEC_X, sc.u0)
@@ -338,8 +338,18 @@ def run_coupled_gpu(flow: Scenario, heat: Scenario, *, precision: int = L.FP64,
         return dict(x_flow=sf.get_vector(L.VEC_X), x_heat=sh.get_vector(L.VEC_X),
                     info_flow=infos_f, info_heat=infos_h, vel=vel)
     finally:
-        sh.close()
-        sf.close()
+        if own:
+            sh.close()
+            sf.close()
+
+
+def make_coupled_systems(flow: Scenario, heat: Scenario, precision: int = L.FP64):
+    pat = L.Pattern(flow.n, flow.mesh.etype, flow.mesh.conn)
+    sf = L.System(pat, precision)
+    sf.set_elements(**flow.element_kwargs())
+    sh = L.System(pat, precision)
+    sh.set_elements(**heat.element_kwargs())
+    return pat, sf, sh
 
 
 def _prepare_step(sc: Scenario, s, k: int, dirichlet: int):
