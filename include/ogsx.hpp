@@ -184,6 +184,15 @@ public:
     {
         check(ogsx_assemble(eqs_.handle(), time_step_length, theta, steady ? 1 : 0), "ogsx_assemble");
     }
+    // CRFProcess::CalIntegrationPointValue -> Cal_Velocity (rf_pcs.cpp:5915,
+    // fem_ele_std.cpp:6554) for this LIQUID_FLOW system's current solution:
+    // the Gauss-point velocities become `heat`'s advection velocity on the
+    // device; `host` (if not null, n_elem*3*ngp doubles) receives them as
+    // [n_elem][3][ngp].
+    void Cal_Velocity(FiniteElementStd& heat, double* host = nullptr)
+    {
+        check(ogsx_gauss_velocity(eqs_.handle(), heat.eqs_.handle(), host), "ogsx_gauss_velocity");
+    }
 
 private:
     LinearEQS& eqs_;
