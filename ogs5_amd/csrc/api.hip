@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "kernels.hpp"
@@ -76,6 +77,12 @@ struct PatternLayout {
     std::vector<uint16_t> sc16;
     bool sell16 = false;
     int64_t npad = 0;
+    // stencil dictionary (formats 13, 14): pattern id per row, padded to
+    // whole slices, and the column offsets of every pattern
+    std::vector<uint8_t> spid;
+    std::vector<int> pdelta;
+    int pat_w = 0, pat_cnt = 0;
+    bool pat_ok = false;
 };
 
 struct ogsx_pattern {
@@ -136,6 +143,10 @@ struct ogsx_system {
     int *sell_ptr = nullptr, *sell_w = nullptr, *sell_cb = nullptr, *sell_c32 = nullptr,
         *sell_src = nullptr;
     uint16_t* sell_c16 = nullptr;
+    bool pat_ok = false;  // stencil dictionary built (formats 13, 14)
+    uint8_t* sell_pid = nullptr;
+    int* pat_delta = nullptr;
+    int pat_w = 0, pat_cnt = 0;
     double* sval64 = nullptr;
     float* sval32 = nullptr;
     uint64_t a_version = 1, sval64_version = 0, sval32_version = 0, a32_version = 0;
@@ -247,6 +258,7 @@ bool valid_fmt(const ogsx_system* s, int f)
     if (f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT || f == kFmtSell16U16 ||
         f == kFmtSell16X2 || f == kFmtSell16U7)
         return s->sell16;
+    if (f == kFmtSellPat8 || f == kFmtSellPat7) return s->pat_ok;
     return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
 }
 
@@ -266,13 +278,15 @@ MatView<VT> mat_view(const ogsx_system* s, const VT* val)
         sv = s->sval32;
     return MatView<VT>{s->n,        s->n_tiles,  s->ptr,      s->col,     s->col16,
                        val,         s->tile_row, s->tile_cb,  s->n_slices, s->sell_ptr,
-                       s->sell_w,   s->sell_cb,  s->sell_c16, s->sell_c32, sv};
+                       s->sell_w,   s->sell_cb,  s->sell_c16, s->sell_c32, sv,
+                       s->sell_pid, s->pat_delta, s->pat_w,   s->pat_cnt};
 }
 
 inline bool is_sell(int f)
 {
     return f == kFmtSell32 || f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT ||
-           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7;
+           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7 || f == kFmtSellPat8 ||
+           f == kFmtSellPat7;
 }
 
 // SELL-64 layout of the pattern: slice widths, offsets, columns (32-bit and,
@@ -323,6 +337,58 @@ void build_sell(const PatternHost& h, std::vector<int>& sptr, std::vector<int>& 
     if (fits16)
         for (int64_t sl = 0; sl < ns; sl++)
             for (int64_t p = sptr[sl]; p < sptr[sl + 1]; p++) c16[p] = (uint16_t)(c32[p] - scb[sl]);
+}
+
+// Stencil dictionary of the pattern: rows grouped by the offsets of their
+// columns from the row index (in CRS order).  Applicable when at most 255
+// distinct offset lists occur and they fit the LDS stage (kPatLds entries);
+// one more all-zero entry serves the padding rows of the last slice.
+void build_pattern_dict(const PatternHost& h, const std::vector<int>& sw, PatternLayout& L)
+{
+    L.pat_ok = false;
+    L.spid.clear();
+    L.pdelta.clear();
+    const int64_t ns = (int64_t)sw.size();
+    int wmax = 0;
+    for (int w : sw) wmax = std::max(wmax, w);
+    if (h.n == 0 || wmax == 0) return;
+    std::map<std::vector<int>, int> ids;
+    std::vector<std::vector<int>> pats;
+    std::vector<uint8_t> pid(ns * kSellC, 0);
+    std::vector<int> key, prev;
+    int prev_id = -1;
+    for (int64_t r = 0; r < h.n; r++) {
+        key.clear();
+        for (int64_t k = h.ptr[r]; k < h.ptr[r + 1]; k++) key.push_back((int)(h.col[k] - r));
+        int id;
+        if (prev_id >= 0 && key == prev) {
+            id = prev_id;
+        } else {
+            auto it = ids.find(key);
+            if (it == ids.end()) {
+                if (pats.size() >= 255) return;
+                id = (int)pats.size();
+                ids.emplace(key, id);
+                pats.push_back(key);
+            } else {
+                id = it->second;
+            }
+            prev = key;
+            prev_id = id;
+        }
+        pid[r] = (uint8_t)id;
+    }
+    const int pad_id = (int)pats.size();  // all-zero offsets: padding rows
+    pats.emplace_back();
+    if ((int64_t)pats.size() * wmax > kPatLds) return;
+    for (int64_t r = h.n; r < ns * kSellC; r++) pid[r] = (uint8_t)pad_id;
+    L.pdelta.assign(pats.size() * wmax, 0);
+    for (size_t p = 0; p < pats.size(); p++)
+        for (size_t j = 0; j < pats[p].size(); j++) L.pdelta[p * wmax + j] = pats[p][j];
+    L.spid.swap(pid);
+    L.pat_w = wmax;
+    L.pat_cnt = (int)pats.size();
+    L.pat_ok = true;
 }
 
 // Brings the SELL value copies up to date with A (after assembly / BC).
@@ -391,6 +457,7 @@ void build_layout(const PatternHost& h, PatternLayout& L)
     if (!L.tile16) L.tc16.clear();
     build_sell(h, L.sptr, L.sw, L.scb, L.sc32, L.sc16, L.ssrc, L.sell16);
     L.npad = L.sptr.back();
+    build_pattern_dict(h, L.sw, L);
 }
 
 // Device bytes a system on pattern p allocates: ogsx_system_create plus
@@ -414,6 +481,7 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
     t += b((int64_t)L.sptr.size(), 4) + b((int64_t)L.sw.size(), 4) + b((int64_t)L.scb.size(), 4) +
          b((int64_t)L.ssrc.size(), 4) + b(L.npad, 8) +
          (precision == OGSX_FP32_IR ? b(L.npad, 4) : 0) + (L.sell16 ? b((int64_t)L.sc16.size(), 2) : 0);
+    if (L.pat_ok) t += b((int64_t)L.spid.size(), 1) + b((int64_t)L.pdelta.size(), 4);
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
          b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
@@ -473,6 +541,8 @@ int prof_flush(ogsx_system* s)
         case kFmtSell16U16: { constexpr int L = kFmtSell16U16; CALL; } break; \
         case kFmtSell16X2: { constexpr int L = kFmtSell16X2; CALL; } break; \
         case kFmtSell16U7: { constexpr int L = kFmtSell16U7; CALL; } break; \
+        case kFmtSellPat8: { constexpr int L = kFmtSellPat8; CALL; } break; \
+        case kFmtSellPat7: { constexpr int L = kFmtSellPat7; CALL; } break; \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -504,8 +574,8 @@ template <typename VT, typename T>
 int tune_one(ogsx_system* s, int* fmt_out)
 {
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
-                         kFmtSell16X2,  kFmtSell16U7, kFmtSell32,   kFmtTile16,
-                         kFmtTile32};
+                         kFmtSell16X2,  kFmtSell16U7, kFmtSellPat8, kFmtSellPat7,
+                         kFmtSell32,    kFmtTile16,   kFmtTile32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -556,6 +626,9 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
                       fmt == kFmtSell16U7)
                          ? 2
                          : 4;
+    if (fmt == kFmtSellPat8 || fmt == kFmtSellPat7)  // values, slice table, one id per row
+        return (double)s->npad * sval + 8.0 * s->n_slices + (double)s->n_slices * kSellC +
+               2 * n * sx;
     if (is_sell(fmt))  // stored entries incl. padding, slice table (ptr, width, base)
         return (double)s->npad * (sval + sidx) + 12.0 * s->n_slices + 2 * n * sx;
     const double tiles = (fmt == kFmtTile16 || fmt == kFmtTile32) ? 8.0 * s->n_tiles : 0.0;
@@ -783,6 +856,17 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
             up(s->sell_src, Ly.ssrc.data(), 4 * Ly.ssrc.size()) ||
             (s->sell16 && up(s->sell_c16, Ly.sc16.data(), 2 * Ly.sc16.size())))
             return cleanup(fail(OGSX_EHIP, "upload of the SELL layout failed"));
+        if (Ly.pat_ok) {
+            if ((rc = dalloc(s, &s->sell_pid, Ly.spid.size())) ||
+                (rc = dalloc(s, &s->pat_delta, Ly.pdelta.size())))
+                return cleanup(rc);
+            if (up(s->sell_pid, Ly.spid.data(), Ly.spid.size()) ||
+                up(s->pat_delta, Ly.pdelta.data(), 4 * Ly.pdelta.size()))
+                return cleanup(fail(OGSX_EHIP, "upload of the stencil dictionary failed"));
+            s->pat_ok = true;
+            s->pat_w = Ly.pat_w;
+            s->pat_cnt = Ly.pat_cnt;
+        }
     }
     // the 32-bit-column SELL and tiled layouts are tuner candidates only when
     // 16-bit slice offsets do not fit; otherwise they are built on request
@@ -1500,8 +1584,9 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
     }
     if (!valid_fmt(s, lanes_per_row))
         return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
-                                 "3/5/6/7/10/11/12 (SELL-64), or 4, 8, 16, 32, 64 lanes per row "
-                                 "(16-bit forms only when the pattern allows)");
+                                 "3/5/6/7/10/11/12 (SELL-64), 13/14 (SELL-64 with a stencil "
+                                 "dictionary), or 4, 8, 16, 32, 64 lanes per row (16-bit and "
+                                 "dictionary forms only when the pattern allows)");
     if (needs_extra_layouts(lanes_per_row) && !s->extra_layouts) {
         OK_TRY(alloc_extra_layouts(s));
         // the matrix views of the device descriptor now hold the new arrays

@@ -175,6 +175,12 @@ struct MatView {
     const uint16_t* sell_c16;
     const int* sell_c32;
     const VT* sval;
+    // stencil dictionary: row r's columns are r + pat_delta[sell_pid[r] *
+    // pat_w + j] (rows padded to the slice, padding rows use the last row)
+    const uint8_t* sell_pid;
+    const int* pat_delta;
+    int pat_w;
+    int pat_cnt;
 };
 
 // MatView with global-address-space pointers, as the device loops use it.
@@ -195,6 +201,10 @@ struct GMat {
     const OGSX_GL uint16_t* sell_c16;
     const OGSX_GL int* sell_c32;
     const OGSX_GL VT* sval;
+    const OGSX_GL uint8_t* sell_pid;
+    const OGSX_GL int* pat_delta;
+    int pat_w;
+    int pat_cnt;
 };
 
 template <typename VT>
@@ -203,7 +213,8 @@ __device__ __forceinline__ GMat<VT> gmat(const MatView<VT>& m)
     return GMat<VT>{m.n,          m.n_tiles,       gl(m.ptr),      gl(m.col),
                     gl(m.col16),  gl(m.val),       gl(m.tile_row), gl(m.tile_cb),
                     m.n_slices,   gl(m.sell_ptr),  gl(m.sell_w),   gl(m.sell_cb),
-                    gl(m.sell_c16), gl(m.sell_c32), gl(m.sval)};
+                    gl(m.sell_c16), gl(m.sell_c32), gl(m.sval),    gl(m.sell_pid),
+                    gl(m.pat_delta), m.pat_w,       m.pat_cnt};
 }
 
 constexpr int kTileU = 8;                 // nonzeros per thread per tile
@@ -217,7 +228,10 @@ constexpr int kFmtSell16NT = 7;           // ... and non-temporal matrix loads
 constexpr int kFmtSell16U16 = 10;         // SELL-64, 16-bit, 16 in flight, NT
 constexpr int kFmtSell16X2 = 11;          // SELL-64, 16-bit, two slices per wave, NT
 constexpr int kFmtSell16U7 = 12;          // SELL-64, 16-bit, 7 entries in flight, NT
+constexpr int kFmtSellPat8 = 13;          // SELL-64, stencil dictionary, 8 in flight, NT
+constexpr int kFmtSellPat7 = 14;          // SELL-64, stencil dictionary, 7 in flight, NT
 constexpr int kSellC = 64;                // rows per slice = wavefront size
+constexpr int kPatLds = 4096;             // dictionary entries staged in LDS
 
 // Tiled CRS SpMV: a block streams the contiguous nonzeros of one tile of
 // rows (<= kTileNnz nonzeros, <= kBlock rows) with kTileU independent
@@ -362,6 +376,60 @@ __device__ __forceinline__ void spmv_sell(const GMat<VT>& m, const OGSX_GL XT* _
     }
 }
 
+// SELL-64 with a stencil dictionary instead of column indices: rows whose
+// columns sit at the same offsets from the row (the interior rows of a
+// structured mesh, and each kind of boundary row) share one dictionary
+// entry, so a row costs one byte (its pattern id) instead of 2-4 bytes per
+// stored entry.  The block stages the dictionary in LDS; lane l of a slice
+// reads entry j's column as row + delta[pid][j].  Values and summation are
+// those of the other SELL formats (same value array, CRS order, rounded
+// products), so results are bit-identical.  Padding entries have value 0
+// and delta 0; lanes past the last row take the last row's column.
+template <int U, bool NT, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_sell_pat(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                              int G, int b, Epi&& epi)
+{
+#pragma clang fp contract(off)
+    __shared__ int pd[kPatLds];
+    const int tot = m.pat_cnt * m.pat_w;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) pd[i] = m.pat_delta[i];
+    __syncthreads();
+    int sb, se;
+    chunk_of(m.n_slices, G, b, sb, se);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int sl = sb + wave; sl < se; sl += kBlock / 64) {
+        const int row = sl * kSellC + lane;
+        const int rowc = row < m.n ? row : m.n - 1;
+        const int base = m.sell_ptr[sl] + lane;
+        const int w = m.sell_w[sl];
+        const int* pr = pd + (int)m.sell_pid[row] * m.pat_w;
+        ACC s = 0;
+        int j = 0;
+        for (; j + U <= w; j += U) {
+            int c[U];
+            VT v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                v[u] = ld<NT>(m.sval + base + (j + u) * kSellC);
+                c[u] = rowc + pr[j + u];
+            }
+            ACC xv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) xv[u] = (ACC)x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const ACC prod = (ACC)v[u] * xv[u];
+                s = s + prod;
+            }
+        }
+        for (; j < w; j++) {
+            const ACC prod = (ACC)m.sval[base + j * kSellC] * (ACC)x[rowc + pr[j]];
+            s = s + prod;
+        }
+        if (row < m.n) epi(row, s);
+    }
+}
+
 // Two slices per wavefront in one loop: twice the independent loads in
 // flight; each row is still summed in CRS order.
 template <int U, typename VT, typename XT, typename ACC, typename Epi>
@@ -445,6 +513,10 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
         spmv_sell<1, 16, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16U7)
         spmv_sell<1, 7, true, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSellPat8)
+        spmv_sell_pat<8, true, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSellPat7)
+        spmv_sell_pat<7, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16X2)
         spmv_sell_x2<4, VT, XT, ACC>(m, x, G, b, epi);
     else

@@ -96,7 +96,7 @@ def test_assembly_parity(name, bc):
     assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
 
 
-@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 4, 8, 16, 32, 64])
 def test_spmv_parity_all_variants(lpr):
     sc = SCEN["hex"]
     A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_ROW)
@@ -106,7 +106,7 @@ def test_spmv_parity_all_variants(lpr):
     assert s.spmv_variant() == lpr
     eq.a[:] = A  # same matrix on both sides
     yr = eq.spmv(x)
-    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12):
+    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14):
         # CRS-order summation over rounded products: the reference's multiVec bits
         assert np.array_equal(y, yr)
         return
@@ -114,6 +114,35 @@ def test_spmv_parity_all_variants(lpr):
     scale = np.zeros(sc.n)
     np.add.at(scale, rows, np.abs(A * x[eq.col]))
     assert np.all(np.abs(y - yr) <= 1e-14 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("name", ["quad", "tet", "hex_aniso_grav", "bmskel"])
+@pytest.mark.parametrize("fmt", [13, 14])
+def test_spmv_stencil_dictionary_bit_exact(name, fmt):
+    """The dictionary formats on 2-D, tet and small meshes (padding rows in
+    the last slice): the reference's multiVec bits."""
+    sc = SCEN[name]
+    A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_SYMMETRIC)
+    s.set_spmv_variant(fmt)
+    x = np.random.default_rng(fmt).standard_normal(sc.n)
+    y = s.spmv(x)
+    eq.a[:] = A
+    assert np.array_equal(y, eq.spmv(x))
+
+
+def test_stencil_dictionary_needs_repeated_patterns():
+    """A pattern with more than 255 distinct column-offset lists has no
+    dictionary: the formats are refused, the others still work."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    conn = np.stack([np.arange(n), rng.integers(0, n, n)], 1)
+    conn = conn[conn[:, 0] != conn[:, 1]]
+    pat = ox.Pattern(n, ox.LINE2, conn)
+    s = ox.System(pat, ox.FP64)
+    with pytest.raises(ox.OgsxError):
+        s.set_spmv_variant(13)
+    s.set_spmv_variant(12)
+    s.close()
 
 
 def _tight_reference(sc):
