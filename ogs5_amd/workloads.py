@@ -262,13 +262,17 @@ def request_scenario(spec: RequestSpec, mesh: Optional[Mesh] = None) -> Scenario
 
 def bmskel() -> Scenario:
     """The reference's tests/data/bmskel deck: one line element of length 1,
-    HEAT_TRANSPORT, T0 = 273, Dirichlet 274 at POINT0 (node 0), one step of
-    dt = 1 with theta = 0.5 and BiCGStab + Jacobi to 1e-12.  Solid density
-    2000, capacity 900, conductivity 5 (a.msp)."""
+    HEAT_TRANSPORT, Dirichlet 274 at POINT0 (node 0), one step of dt = 1 with
+    theta = 0.5 and BiCGStab + Jacobi to 1e-12.  Solid density 2000, capacity
+    900, conductivity 5 (a.msp).  The initial condition 273 applies to POINT0
+    only: a.ic lists "POINT POINT0" and "POINT POINT1" under one $GEO_TYPE,
+    and the reference reads the first line only (FEM/rf_ic_new.cpp:363-387),
+    so node 1 starts at 0 (ogs5_amd.deck reads the deck files the same way)."""
     m = line_mesh(1, 1.0)
     return Scenario("bmskel", m, L.HEAT_TRANSPORT, np.array([2000.0 * 900.0]), np.array([5.0]), 1,
                     np.array([0], np.int64), np.array([274.0]), np.zeros(0, np.int64),
-                    np.zeros(0), np.full(2, 273.0), [Step(1.0, 0.5)], tol=1e-12, max_iter=1000)
+                    np.zeros(0), np.array([273.0, 0.0]), [Step(1.0, 0.5)], tol=1e-12,
+                    max_iter=1000)
 
 
 # ---------------------------------------------------------------------------
