@@ -127,8 +127,57 @@ def _model_value(data: List[str], what: str) -> float:
 # ---------------------------------------------------------------------------
 # mesh and geometry
 # ---------------------------------------------------------------------------
+def _msh_bulk(raw: bytes):
+    """The $NODES and $ELEMENTS blocks parsed by numpy's C text parser when
+    the mesh is plain (one mesh, one element type, no comments or $AREA
+    lines inside the blocks); None otherwise (the line reader takes over)."""
+    import warnings
+    i0 = raw.find(b"#FEM_MSH")
+    if i0 < 0 or raw.find(b"#FEM_MSH", i0 + 1) >= 0:
+        return None
+
+    def block(key):
+        k = raw.find(key, i0)
+        if k < 0:
+            return None, None
+        a = raw.index(b"\n", k) + 1
+        b = raw.index(b"\n", a)
+        count = int(raw[a:b].split()[0])
+        ends = [e for e in (raw.find(b"$", b), raw.find(b"#", b)) if e >= 0]
+        return count, raw[b:min(ends) if ends else len(raw)]
+
+    n, nb = block(b"$NODES")
+    E, eb = block(b"$ELEMENTS")
+    if nb is None or eb is None or b";" in nb or b";" in eb:
+        return None
+    first = eb.split(b"\n", 2)[1].split() if E else []
+    if len(first) < 4:
+        return None
+    j = 3 if first[2] == b"-1" else 2
+    name = first[j].decode()
+    if name not in _ETYPES:
+        raise DeckError(f"element type '{name}' is not supported")
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        xyz = np.fromstring(nb, dtype=np.float64, sep=" ")
+        el = np.fromstring(eb.replace(b" " + first[j] + b" ", b" "), dtype=np.int64, sep=" ")
+    ncol = len(first) - 1
+    if xyz.size != 4 * n or el.size != E * ncol:
+        return None
+    xyz = xyz.reshape(n, 4)
+    el = el.reshape(E, ncol)
+    if not np.array_equal(xyz[:, 0], np.arange(n)):
+        raise DeckError("node ids must be 0..n-1 in order")
+    return xyz[:, 1:4].copy(), _ETYPES[name], np.ascontiguousarray(el[:, j:]), el[:, 1].copy()
+
+
 def read_msh(path: str) -> Tuple[Mesh, np.ndarray]:
     """Nodes, elements and material groups of the first #FEM_MSH."""
+    with open(path, "rb") as f:
+        fast = _msh_bulk(f.read())
+    if fast is not None:
+        nodes, etype, elems, groups = fast
+        return _make_mesh(nodes, etype, elems), groups
     lines = _data_lines(path)
     try:
         i = next(k for k, s in enumerate(lines) if s.startswith("#FEM_MSH"))
@@ -176,12 +225,18 @@ def read_msh(path: str) -> Tuple[Mesh, np.ndarray]:
         k += 1
     if nodes is None or elems is None:
         raise DeckError(f"{path}: $NODES and $ELEMENTS are required")
+    return _make_mesh(nodes, etype, elems), groups
+
+
+def _make_mesh(nodes: np.ndarray, etype: int, elems: np.ndarray) -> Mesh:
+    if elems.shape[1] != {LINE2: 2, QUAD4: 4, HEX8: 8, TET4: 4}[etype]:
+        raise DeckError("element node count does not match its type")
     ed = EDIM[etype]
     x, y, z, flag = _plane_coordinates(nodes, ed)
     mesh = Mesh(etype, ed, x, y, z, elems, (), (), np.arange(elems.shape[0]))
     mesh.coordinate_flag = flag
     mesh.xyz = nodes
-    return mesh, groups
+    return mesh
 
 
 def _plane_coordinates(nodes: np.ndarray, edim: int):
