@@ -45,8 +45,14 @@ extern "C" {
 #define OGSX_HEX8 3
 #define OGSX_TET4 5
 
-/* processes (FiniteElement::ProcessType subset, FEM/FEMEnums.h) */
+/* processes (a subset of FiniteElement::ProcessType, FEM/FEMEnums.h; own
+ * numbering) */
 #define OGSX_LIQUID_FLOW 1
+#define OGSX_GROUNDWATER_FLOW 2 /* HEAD form: Laplace K*tuf (CalCoefLaplace,
+                                   fem_ele_std.cpp:2308-2384), mass = storage
+                                   (CalCoefMass :1625-1630), parabolic step as
+                                   LIQUID (:9391), no gravity; mu and rho_g
+                                   are not used */
 #define OGSX_HEAT_TRANSPORT 3
 
 /* Krylov methods: the $LINEAR_SOLVER method ids of Linear_EQS
@@ -121,7 +127,7 @@ int ogsx_system_bytes(const ogsx_system* s, int64_t* bytes);
 /* Element data for CFiniteElementStd::Assembly (FEM/fem_ele_std.cpp:9367).
  * Uploaded once; later assemblies reuse it. */
 typedef struct {
-    int process;          /* OGSX_LIQUID_FLOW / OGSX_HEAT_TRANSPORT           */
+    int process;          /* OGSX_LIQUID_FLOW / _GROUNDWATER_FLOW / _HEAT_TRANSPORT */
     int dim;              /* problem dimension (size of the K tensor)         */
     int n_gauss;          /* $ELE_GAUSS_POINTS, default 3 (rf_num_new.cpp:124) */
     int cond_stride;      /* 1: isotropic scalar per element; dim*dim: tensor */
@@ -157,8 +163,9 @@ int ogsx_system_initialize(ogsx_system* s);
  * zero storage coefficient).  u_old is the system's OGSX_VEC_UOLD vector. */
 int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady);
 
-/* Darcy velocity at the Gauss points of every element of a LIQUID_FLOW
- * system from its current solution (OGSX_VEC_X, the new time level):
+/* Darcy velocity at the Gauss points of every element of a LIQUID_FLOW (or
+ * GROUNDWATER_FLOW: v = -K grad h) system from its current solution
+ * (OGSX_VEC_X, the new time level):
  * CRFProcess::CalIntegrationPointValue (rf_pcs.cpp:5915-5994) ->
  * CFiniteElementStd::Cal_Velocity (fem_ele_std.cpp:6554-6868),
  * v = -(K/mu)(grad p + rho_g e_dim) per Gauss point, in the layout of

@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("OGSX_LIB") or os.path.join(_HERE, "libogsx.so")
 
 OK = 0
 LINE2, QUAD4, HEX8, TET4 = 1, 2, 3, 5
-LIQUID_FLOW, HEAT_TRANSPORT = 1, 3
+LIQUID_FLOW, GROUNDWATER_FLOW, HEAT_TRANSPORT = 1, 2, 3
 BICGSTAB, CG = 2, 5
 PRECOND_NONE, PRECOND_JACOBI = 0, 1
 FP64, FP32_IR = 0, 1

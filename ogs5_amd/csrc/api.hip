@@ -913,14 +913,17 @@ int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e)
     if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     const PatternHost& h = *s->pat;
     const int ed = edim_of(h.etype);
-    if (e->process != OGSX_LIQUID_FLOW && e->process != OGSX_HEAT_TRANSPORT)
+    if (e->process != OGSX_LIQUID_FLOW && e->process != OGSX_GROUNDWATER_FLOW &&
+        e->process != OGSX_HEAT_TRANSPORT)
         return fail(OGSX_EINVAL, "unknown process");
+    const bool head_form = e->process == OGSX_GROUNDWATER_FLOW;
     if (e->dim != ed) return fail(OGSX_EINVAL, "problem dimension must equal the element dimension");
     if (e->cond_stride != 1 && e->cond_stride != ed * ed)
         return fail(OGSX_EINVAL, "cond_stride must be 1 or dim*dim");
     if (!e->x || (ed >= 2 && !e->y) || (ed >= 3 && !e->z) || !e->mass_coef || !e->cond)
         return fail(OGSX_EINVAL, "missing element data");
-    if (e->mu == 0.0 || e->tuf == 0.0) return fail(OGSX_EINVAL, "tuf and mu must be nonzero");
+    if ((!head_form && e->mu == 0.0) || e->tuf == 0.0)
+        return fail(OGSX_EINVAL, "tuf and mu must be nonzero");
     GaussTable t;
     if (build_gauss_table(h.etype, e->n_gauss, t) != OGSX_OK)
         return fail(OGSX_EINVAL, "unsupported n_gauss");
@@ -939,10 +942,12 @@ int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e)
     s->ngp = t.ngp;
     s->cond_stride = e->cond_stride;
     s->tuf = e->tuf;
-    s->mu = e->mu;
+    // the HEAD form is the flow operator with mat = tuf*K/1 (= K*tuf*1 of
+    // the reference, exactly) and no gravity
+    s->mu = head_form ? 1.0 : e->mu;
     s->geo_area = e->geo_area;
     s->adv_coef = e->adv_coef;
-    s->rho_g = e->rho_g;
+    s->rho_g = head_form ? 0.0 : e->rho_g;
     auto up = [&](void* d, const void* src, size_t bytes) {
         return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s->stream);
     };
@@ -987,7 +992,7 @@ template <int ET>
 void launch_assemble_et(ogsx_system* s, const AsmArgs& a)
 {
     const bool iso = s->cond_stride == 1;
-    if (s->process == OGSX_LIQUID_FLOW) {
+    if (s->process != OGSX_HEAT_TRANSPORT) {  // LIQUID_FLOW, GROUNDWATER_FLOW
         if (iso)
             launch_assemble_colours<ET, OGSX_LIQUID_FLOW, true>(s, a);
         else
@@ -1144,8 +1149,9 @@ int ogsx_gauss_velocity(ogsx_system* flow, ogsx_system* heat, double* host_out)
     if (heat == flow) return fail(OGSX_EINVAL, "heat and flow are the same system");
     if (flow->sc.active || (heat && heat->sc.active))
         return fail(OGSX_ESTATE, "system has a solve in progress");
-    if (!flow->elements_set || flow->process != OGSX_LIQUID_FLOW)
-        return fail(OGSX_EINVAL, "flow must be a LIQUID_FLOW system with its elements set");
+    if (!flow->elements_set || flow->process == OGSX_HEAT_TRANSPORT)
+        return fail(OGSX_EINVAL, "flow must be a LIQUID_FLOW or GROUNDWATER_FLOW system with "
+                                 "its elements set");
     const PatternHost& h = *flow->pat;
     const size_t need = (size_t)h.n_elem * 3 * flow->ngp;
     double* dst = nullptr;

@@ -1,4 +1,5 @@
-"""OGS-5 input decks: the subset that drives LIQUID_FLOW and HEAT_TRANSPORT,
+"""OGS-5 input decks: the subset that drives LIQUID_FLOW, GROUNDWATER_FLOW and
+HEAT_TRANSPORT,
 read into the scenarios the device path runs, and a driver for a deck's
 time loop.
 
@@ -32,6 +33,9 @@ blanks skipped, a line starting with ';' is a comment, text after ';' is
 dropped; #STOP ends a file.
 
 Coefficients (per element, from its material group):
+  GROUNDWATER_FLOW  HEAD form: mass = storage, K = the conductivity tensor
+                times tuf, no viscosity, no gravity (fem_ele_std.cpp:1625,
+                2308-2384).
   LIQUID_FLOW   mass = storage (CalCoefMass, FEM/fem_ele_std.cpp:1580-1622,
                 drho/dp = 0); K from the permeability tensor; mu = viscosity;
                 gravity rho*g on the vertical axis when the mesh's coordinate
@@ -63,8 +67,11 @@ from .workloads import Scenario, Step
 
 GRAVITY_CONSTANT = 9.81  # FEM/rf_mfp_new.cpp:49
 _ETYPES = {"line": LINE2, "quad": QUAD4, "hex": HEX8, "tet": TET4}
-_PROCESS = {"LIQUID_FLOW": L.LIQUID_FLOW, "HEAT_TRANSPORT": L.HEAT_TRANSPORT}
-_PRIMARY = {L.LIQUID_FLOW: "PRESSURE1", L.HEAT_TRANSPORT: "TEMPERATURE1"}
+_PROCESS = {"LIQUID_FLOW": L.LIQUID_FLOW, "GROUNDWATER_FLOW": L.GROUNDWATER_FLOW,
+            "HEAT_TRANSPORT": L.HEAT_TRANSPORT}
+_FLOW = ("LIQUID_FLOW", "GROUNDWATER_FLOW")
+_PRIMARY = {L.LIQUID_FLOW: "PRESSURE1", L.GROUNDWATER_FLOW: "HEAD",
+            L.HEAT_TRANSPORT: "TEMPERATURE1"}
 _TIME_UNIT = (("MINUTE", 60.0), ("HOUR", 3600.0), ("DAY", 86400.0),
               ("MONTH", 2592000.0), ("YEAR", 31536000.0))
 
@@ -613,7 +620,7 @@ class Deck:
 
     @property
     def has_flow(self) -> bool:
-        return "LIQUID_FLOW" in self.processes
+        return any(p in _FLOW for p in self.processes)
 
     def scenario(self, pcs: str) -> Scenario:
         """The process's time loop as a Scenario (element coefficients,
@@ -631,12 +638,12 @@ class Deck:
         geo_area = self.media[0].geo_area
         if any(md.geo_area != geo_area for md in self.media):
             raise DeckError("one $GEOMETRY_AREA for all material groups is supported")
-        if proc == L.LIQUID_FLOW:
+        if proc in (L.LIQUID_FLOW, L.GROUNDWATER_FLOW):
             mass = np.array([self.media[k].storage for k in g])
             perms = []
             for md in self.media:
                 if md.perm is None:
-                    raise DeckError("LIQUID_FLOW needs $PERMEABILITY_TENSOR")
+                    raise DeckError(f"{pcs} needs $PERMEABILITY_TENSOR")
                 perms.append(md.perm)
             iso = all(np.array_equal(p, np.eye(D) * p[0, 0]) for p in perms)
             cond = (np.array([perms[k][0, 0] for k in g]) if iso else
@@ -645,10 +652,13 @@ class Deck:
             vertical = m.coordinate_flag % 10 == 2
             rho_g = fl.gravity * fl.density if (vertical and fl.cal_gravity and
                                                 fl.gravity >= 1e-20) else 0.0
-            if fl.viscosity == 0.0:
-                raise DeckError("LIQUID_FLOW needs a nonzero $VISCOSITY")
             theta = num.nls_relaxation
-            extra = dict(mu=fl.viscosity, rho_g=rho_g)
+            if proc == L.GROUNDWATER_FLOW:  # HEAD form: K is the conductivity
+                extra = dict(mu=1.0, rho_g=0.0)
+            else:
+                if fl.viscosity == 0.0:
+                    raise DeckError("LIQUID_FLOW needs a nonzero $VISCOSITY")
+                extra = dict(mu=fl.viscosity, rho_g=rho_g)
         else:
             poro = np.array([self.media[k].porosity for k in g]) if self.has_flow else \
                 np.zeros(m.n_elem)
@@ -742,7 +752,7 @@ def run_deck(base: str, out_dir: Optional[str] = None, precision: int = L.FP64) 
     writer.write(0, t, values)
     infos = {p: [] for p in deck.processes}
     n_steps = max(len(sc.steps) for sc in scs)
-    flow_i = deck.processes.index("LIQUID_FLOW") if deck.has_flow else None
+    flow_i = next((i for i, p in enumerate(deck.processes) if p in _FLOW), None)
     heat_i = deck.processes.index("HEAT_TRANSPORT") if "HEAT_TRANSPORT" in deck.processes else None
     try:
         for k in range(n_steps):

@@ -61,6 +61,8 @@ def lib():
             "orc_element_matrices": (None, [C.POINTER(Problem), C.c_int64, P, P, P, P]),
             "orc_element_system": (None, [C.POINTER(Problem), C.c_int64, P, P]),
             "orc_gauss_velocity": (None, [C.POINTER(Problem), P, P]),
+            "orc_jds_build": (C.c_int64, [C.c_int64, P, P, P, P, P, P]),
+            "orc_jds_spmv": (None, [C.c_int64, C.c_int64, P, P, P, P, P, P]),
             "orc_eqs_init": (None, [C.POINTER(Eqs), C.c_int64, C.c_int64, P, P, P, P, P, P]),
             "orc_eqs_config": (None, [C.POINTER(Eqs), C.c_int, C.c_int, C.c_double, C.c_int]),
             "orc_eqs_initialize": (None, [C.POINTER(Eqs)]),
@@ -227,3 +229,20 @@ def ref_dump_lines():
     if not os.path.exists(REF_DUMP):
         return None
     return subprocess.run([REF_DUMP], check=True, capture_output=True, text=True).stdout.splitlines()
+
+
+def jds_spmv(ptr, col, a, x):
+    """JDS layout of the CRS pattern and y = A x in JDS storage order."""
+    n, nnz = ptr.size - 1, int(ptr[-1])
+    n2o = np.empty(max(n, 1), np.int64)
+    ncol = np.empty(max(n, 1), np.int64)
+    jcol = np.empty(max(nnz, 1), np.int64)
+    src = np.empty(max(nnz, 1), np.int64)
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    maxc = lib().orc_jds_build(n, _p(ptr), _p(col), _p(n2o), _p(ncol), _p(jcol), _p(src))
+    aj = np.ascontiguousarray(np.asarray(a, np.float64)[src[:nnz]])
+    y = np.empty(n)
+    lib().orc_jds_spmv(n, maxc, _p(ncol), _p(n2o), _p(jcol), _p(aj),
+                       _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y, dict(n2o=n2o[:n], ncol=ncol[:maxc], jcol=jcol[:nnz], src=src[:nnz])

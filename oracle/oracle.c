@@ -485,7 +485,9 @@ static void load_coords(const orc_problem* p, int64_t e, double* X, double* Y,
 }
 
 /* mat[dim*dim] = CalCoefLaplace (fem_ele_std.cpp:2202): LIQUID
- * tuf*K/mu (:2298-2301); HEAT the conductivity tensor (:2538-2543). */
+ * tuf*K/mu (:2298-2301); GROUNDWATER_FLOW K*tuf*k_rel with k_rel = 1
+ * (:2308-2384, hydraulic conductivity, no viscosity); HEAT the
+ * conductivity tensor (:2538-2543). */
 static void laplace_tensor(const orc_problem* p, int64_t e, double* mat)
 {
     const int dim = p->dim;
@@ -500,6 +502,8 @@ static void laplace_tensor(const orc_problem* p, int64_t e, double* mat)
     if (p->process == ORC_LIQUID_FLOW) {
         for (int i = 0; i < dim * dim; i++)
             mat[i] = p->tuf * tensor[i] / p->mu * 1.0 * 1.0;
+    } else if (p->process == ORC_GROUNDWATER_FLOW) {
+        for (int i = 0; i < dim * dim; i++) mat[i] = tensor[i] * p->tuf * 1.0;
     } else {
         for (int i = 0; i < dim * dim; i++) mat[i] = tensor[i];
     }
@@ -609,7 +613,8 @@ void orc_gauss_velocity(const orc_problem* p, const double* u, double* vel)
                 g[i] = 0.0;
                 for (int j = 0; j < npe; j++) g[i] += nodal[j] * dshp[i * npe + j];
             }
-            if (p->rho_g != 0.0) g[dim - 1] += p->rho_g;
+            /* HEAD form (GROUNDWATER_FLOW): HEAD_Flag, no gravity term */
+            if (p->rho_g != 0.0 && p->process == ORC_LIQUID_FLOW) g[dim - 1] += p->rho_g;
             for (int i = 0; i < dim; i++)
                 for (int j = 0; j < dim; j++)
                     out[i * ngp + gp] -= mat[dim * i + j] * g[j] / p->tuf;
@@ -638,8 +643,9 @@ static void element_system_full(const orc_problem* p, int64_t e, double* lhs,
     orc_element_matrices(p, e, M, L, A, G);
     const double dt_inverse = 1.0 / p->dt;
     for (int a = 0; a < npe; a++) u0[a] = p->u_old[p->conn[e * npe + a]];
-    if (p->process == ORC_LIQUID_FLOW) {
-        /* AssembleParabolicEquation (:7702): relax0 = theta, relax1 = 1-theta */
+    if (p->process != ORC_HEAT_TRANSPORT) {
+        /* AssembleParabolicEquation (:7702; GROUNDWATER_FLOW too, :9391):
+         * relax0 = theta, relax1 = 1-theta */
         double relax0 = p->theta;
         if (relax0 < DBL_MIN) relax0 = 1.0;
         const double relax1 = 1.0 - relax0;
@@ -797,6 +803,59 @@ void orc_spmv(const orc_eqs* q, const double* x, double* y)
     for (int64_t i = 0; i < q->n; i++)
         for (int64_t k = q->ptr[i]; k < q->ptr[i + 1]; k++)
             y[i] += q->a[k] * x[q->col[k]];
+}
+
+/* SparseTable in JDS storage (matrix_class.cpp:778-861): rows sorted by
+ * decreasing length with the reference's insertion sort (equal lengths
+ * keep their order), then column k of the jagged table holds the k-th
+ * connected node of every row that has one.  Returns max_columns; n2o is
+ * row_index_mapping_n2o, ncol num_column_entries[max_columns], jcol
+ * entry_column, src the CRS position of each JDS entry (for its value). */
+int64_t orc_jds_build(int64_t n, const int64_t* ptr, const int64_t* col, int64_t* n2o,
+                      int64_t* ncol, int64_t* jcol, int64_t* src)
+{
+    int64_t* len = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) {
+        n2o[i] = i;
+        len[i] = ptr[i + 1] - ptr[i];
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t l0 = len[i], r0 = n2o[i];
+        int64_t j = i;
+        while (j > 0 && len[j - 1] < l0) {
+            len[j] = len[j - 1];
+            n2o[j] = n2o[j - 1];
+            j--;
+        }
+        len[j] = l0;
+        n2o[j] = r0;
+    }
+    const int64_t maxc = n > 0 ? len[0] : 0;
+    for (int64_t k = 0; k < maxc; k++) ncol[k] = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t k = 0; k < len[i]; k++) ncol[k]++;
+    int64_t c = 0;
+    for (int64_t k = 0; k < maxc; k++)
+        for (int64_t i = 0; i < ncol[k]; i++) {
+            const int64_t ii = n2o[i];
+            jcol[c] = col[ptr[ii] + k];
+            src[c] = ptr[ii] + k;
+            c++;
+        }
+    free(len);
+    return maxc;
+}
+
+/* CSparseMatrix::multiVec for JDS, DOF = 1 (matrix_class.cpp:1501-1525
+ * pattern): y = 0, then per jagged column k and row slot i,
+ * y[n2o[i]] += a[c] * x[jcol[c]]. */
+void orc_jds_spmv(int64_t n, int64_t maxc, const int64_t* ncol, const int64_t* n2o,
+                  const int64_t* jcol, const double* a, const double* x, double* y)
+{
+    for (int64_t i = 0; i < n; i++) y[i] = 0.0;
+    int64_t c = 0;
+    for (int64_t k = 0; k < maxc; k++)
+        for (int64_t i = 0; i < ncol[k]; i++, c++) y[n2o[i]] += a[c] * x[jcol[c]];
 }
 
 void orc_precond_jacobi(const orc_eqs* q, const double* r, double* z)

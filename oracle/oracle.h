@@ -28,7 +28,7 @@ extern "C" {
 /* Element types, numbered as MshElemType::type (MSH/MshElemType.h). */
 enum { ORC_LINE2 = 1, ORC_QUAD4 = 2, ORC_HEX8 = 3, ORC_TET4 = 5 };
 /* Process types on the path. */
-enum { ORC_LIQUID_FLOW = 1, ORC_HEAT_TRANSPORT = 3 };
+enum { ORC_LIQUID_FLOW = 1, ORC_GROUNDWATER_FLOW = 2, ORC_HEAT_TRANSPORT = 3 };
 
 /* ---- a4: Gauss tables and shape functions (FEM/mathlib.cpp) ---------- */
 double orc_gauss_pkt(long grd, long pkt);           /* mathlib.cpp:1112 */
@@ -58,7 +58,8 @@ void orc_pattern_fill(int64_t n_nodes, int64_t n_elem, int npe,
 /* ---- a5..a16: element assembly ----------------------------------------- */
 typedef struct {
     int etype;          /* ORC_LINE2 / QUAD4 / HEX8 / TET4                    */
-    int process;        /* ORC_LIQUID_FLOW or ORC_HEAT_TRANSPORT               */
+    int process;        /* ORC_LIQUID_FLOW, ORC_GROUNDWATER_FLOW (HEAD form) or
+                           ORC_HEAT_TRANSPORT                                  */
     int dim;            /* problem (mesh) dimension: tensor is dim x dim       */
     int n_gauss;        /* $ELE_GAUSS_POINTS (default 3)                       */
     int64_t n_nodes, n_elem;
@@ -99,6 +100,13 @@ void orc_element_system(const orc_problem* p, int64_t e, double* lhs,
  * CRFProcess::CalIntegrationPointValue rf_pcs.cpp:5966-5993 does):
  * vel[(e*3+i)*ngp+gp] = ElementValue::Velocity(i, gp). */
 void orc_gauss_velocity(const orc_problem* p, const double* u, double* vel);
+
+/* JDS storage of the sparse table (matrix_class.cpp:778-861) and its SpMV
+ * (:1501); see oracle.c. */
+int64_t orc_jds_build(int64_t n, const int64_t* ptr, const int64_t* col, int64_t* n2o,
+                      int64_t* ncol, int64_t* jcol, int64_t* src);
+void orc_jds_spmv(int64_t n, int64_t maxc, const int64_t* ncol, const int64_t* n2o,
+                  const int64_t* jcol, const double* a, const double* x, double* y);
 
 /* ---- Linear_EQS (FEM/equation_class.h:56) ------------------------------- */
 typedef struct {

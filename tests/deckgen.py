@@ -242,10 +242,18 @@ FILES = {
 }
 
 
-def write_section_deck(directory, base="sec"):
-    """Writes the deck; returns its base path."""
+def write_section_deck(directory, base="sec", flow="LIQUID_FLOW"):
+    """Writes the deck; flow = "GROUNDWATER_FLOW" makes the flow process
+    the HEAD form (primary variable HEAD, the tensor is the hydraulic
+    conductivity, boundary heads in m).  Returns the deck's base path."""
     os.makedirs(directory, exist_ok=True)
     for ext, text in FILES.items():
+        if flow == "GROUNDWATER_FLOW":
+            text = (text.replace("LIQUID_FLOW", "GROUNDWATER_FLOW").replace("PRESSURE1", "HEAD")
+                    .replace("CONSTANT 2.0e5", "CONSTANT 20.0")
+                    .replace("CONSTANT 1.0e5", "CONSTANT 10.0")
+                    .replace("ORTHOTROPIC 1.0e-13 2.0e-14", "ORTHOTROPIC 1.0e-6 2.0e-7")
+                    .replace("ISOTROPIC 5.0e-14", "ISOTROPIC 5.0e-7"))
         with open(os.path.join(directory, base + ext), "w") as f:
             f.write(text)
     with open(os.path.join(directory, base + ".msh"), "w") as f:

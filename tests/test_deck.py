@@ -82,6 +82,19 @@ def test_section_deck(tmp_path):
                      ("DOMAIN", "VTK", None, [3.0])]
 
 
+def test_groundwater_section_deck(tmp_path):
+    d = D.read_deck(write_section_deck(str(tmp_path), flow="GROUNDWATER_FLOW"))
+    assert d.processes == ["GROUNDWATER_FLOW", "HEAT_TRANSPORT"] and d.has_flow
+    gw, heat = d.scenario("GROUNDWATER_FLOW"), d.scenario("HEAT_TRANSPORT")
+    assert gw.process == 2 and gw.mu == 1.0 and gw.rho_g == 0.0  # HEAD form
+    assert gw.meta["variable"] == "HEAD" and gw.tuf == 86400.0
+    K = gw.cond.reshape(-1, 2, 2)
+    assert np.array_equal(K[d.groups == 0][0], np.diag([1e-6, 2e-7]))
+    assert np.all(gw.u0 == 10.0) and set(gw.bc_val) == {20.0, 10.0}
+    n = np.where(d.groups == 0, 0.2, 0.3)
+    assert np.array_equal(heat.cond, np.where(d.groups == 0, 2.5, 3.0) * (1.0 - n) + n * 0.6)
+
+
 def test_unsupported_keywords_are_refused(tmp_path):
     base = write_section_deck(str(tmp_path))
     with open(base + ".mmp") as f:

@@ -35,14 +35,16 @@ def test_bmskel_deck_end_to_end(tmp_path):
     assert res["infos"]["HEAT_TRANSPORT"][0]["converged"] == 1
 
 
-def test_section_deck_matches_oracle(tmp_path):
+@pytest.mark.parametrize("flow_pcs,var", [("LIQUID_FLOW", "PRESSURE1"),
+                                          ("GROUNDWATER_FLOW", "HEAD")])
+def test_section_deck_matches_oracle(tmp_path, flow_pcs, var):
     from oracle import ffi
-    base = write_section_deck(str(tmp_path))
+    base = write_section_deck(str(tmp_path), flow=flow_pcs)
     res = D.run_deck(base)
     d = res["deck"]
-    flow, heat = d.scenario("LIQUID_FLOW"), d.scenario("HEAT_TRANSPORT")
+    flow, heat = d.scenario(flow_pcs), d.scenario("HEAT_TRANSPORT")
     ref = driver.run_oracle_coupled(flow, heat, ffi, tol=1e-14)
-    p, T = res["values"]["PRESSURE1"], res["values"]["TEMPERATURE1"]
+    p, T = res["values"][var], res["values"]["TEMPERATURE1"]
     print(f"[deck] section p in [{p.min():.4g}, {p.max():.4g}], T in [{T.min():.4g}, "
           f"{T.max():.4g}]", flush=True)
     for got, want in ((p, ref["x_flow"]), (T, ref["x_heat"])):
@@ -50,7 +52,7 @@ def test_section_deck_matches_oracle(tmp_path):
     assert all(i["converged"] == 1 for v in res["infos"].values() for i in v)
     assert T.max() > 10.5  # the inlet heat moved in
     names = sorted(os.path.basename(f) for f in res["files"])
-    assert names == ["sec_LIQUID_FLOW0003.vtk", "sec_domain_HEAT_TRANSPORT_quad.tec",
+    assert names == [f"sec_{flow_pcs}0003.vtk", "sec_domain_HEAT_TRANSPORT_quad.tec",
                      "sec_time_OBS.tec"]
     curve = (tmp_path / "sec_time_OBS.tec").read_text().splitlines()
     assert len(curve) == 3 + 4  # header, t = 0, 1, 2, 3

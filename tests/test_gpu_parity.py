@@ -39,6 +39,11 @@ def _scenarios():
     sc.cond_stride = 9
     sc.rho_g = 1000.0 * 9.81
     out["hex_aniso_grav"] = sc
+    # GROUNDWATER_FLOW (HEAD form): viscosity and density set but not used
+    gw = W.quad_flow(20, 16, seed=9, n_steps=3)
+    gw.process, gw.tuf, gw.mu, gw.rho_g = ox.GROUNDWATER_FLOW, 86400.0, 5.0, 9.81e3
+    gw.steps = [W.Step(st.dt / 86400.0, st.theta) for st in gw.steps]
+    out["quad_gw"] = gw
     return out
 
 
@@ -174,7 +179,7 @@ def test_reference_solution_chain_is_exact():
         assert np.linalg.norm(ref["steps"][0] - x) / np.linalg.norm(x) < 1e-12
 
 
-@pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav"])
+@pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav", "quad_gw"])
 @pytest.mark.parametrize("method,bc,prec", [
     (ox.CG, ox.DIRICHLET_SYMMETRIC, ox.FP64),
     (ox.BICGSTAB, ox.DIRICHLET_ROW, ox.FP64),

@@ -39,7 +39,7 @@ def _mesh(kind, rng):
     return m
 
 
-def _flow_pair(m, rng, *, tensor, n_gauss, rho_g, tuf=1.0, mu=1.3e-3):
+def _flow_pair(m, rng, *, tensor, n_gauss, rho_g, tuf=1.0, mu=1.3e-3, process=None):
     """GPU LIQUID_FLOW system and the oracle problem on the same data, with a
     random nodal pressure in the system's x vector."""
     from oracle import ffi
@@ -51,7 +51,7 @@ def _flow_pair(m, rng, *, tensor, n_gauss, rho_g, tuf=1.0, mu=1.3e-3):
     else:
         cond, stride = k, 1
     u = 1e5 * rng.standard_normal(m.n_nodes)
-    kw = dict(process=ox.LIQUID_FLOW, mass_coef=np.full(m.n_elem, 1e-7), cond=cond,
+    kw = dict(process=process or ox.LIQUID_FLOW, mass_coef=np.full(m.n_elem, 1e-7), cond=cond,
               cond_stride=stride, n_gauss=n_gauss, tuf=tuf, mu=mu, rho_g=rho_g)
     pat = ox.Pattern(m.n_nodes, m.etype, m.conn)
     s = ox.System(pat, ox.FP64)
@@ -82,6 +82,18 @@ def test_velocity_matches_oracle(kind, tensor, n_gauss, gravity):
     assert v.shape == vr.shape
     assert np.abs(v - vr).max() <= VEL_TOL * np.abs(vr).max()
     assert not v[:, m.dim:, :].any()
+    s.close()
+
+
+def test_groundwater_velocity_matches_oracle():
+    """HEAD form: v = -K grad h, gravity and viscosity not used."""
+    rng = np.random.default_rng(22)
+    m = _mesh("hex", rng)
+    pat, s, prob, u = _flow_pair(m, rng, tensor=True, n_gauss=2, rho_g=9.81e3, tuf=86400.0,
+                                 mu=3.0, process=ox.GROUNDWATER_FLOW)
+    v = s.gauss_velocity()
+    vr = prob.gauss_velocity(u)
+    assert np.abs(v - vr).max() <= VEL_TOL * np.abs(vr).max()
     s.close()
 
 

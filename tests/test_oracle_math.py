@@ -317,3 +317,51 @@ def test_reference_bicgstab_breakdown_on_dirichlet_only_rhs(ffi):
     assert r["info"][0]["ret"] == 0 and r["info"][0]["error"] > 0.1
     c = driver.run_oracle(sc, ffi, method=5, symmetric=True, tol=1e-13)
     assert c["info"][0]["ret"] == 1 and c["info"][0]["error"] < 1e-13
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_jds_storage_sums_like_crs(ffi, seed):
+    """The reference's JDS storage (rows sorted by decreasing length, jagged
+    columns) sums every row in the same order as CRS, so its SpMV equals the
+    CRS SpMV bit for bit: the device path, which keeps CRS order, serves
+    decks with storage 4 unchanged."""
+    rng = np.random.default_rng(seed)
+    m = [M.tet_mesh(3, 2, 2), M.quad_mesh(5, 4), M.hex_mesh(2, 3, 2)][seed]
+    ptr, col, diag = ffi.pattern(m.n_nodes, m.conn)
+    a = rng.standard_normal(int(ptr[-1]))
+    x = rng.standard_normal(m.n_nodes)
+    eq = ffi.OracleEqs(ptr, col, diag)
+    eq.a[:] = a
+    y_crs = eq.spmv(x)
+    y_jds, lay = ffi.jds_spmv(ptr, col, a, x)
+    assert np.array_equal(y_jds, y_crs)
+    lens = np.diff(ptr)[lay["n2o"]]
+    assert np.all(np.diff(lens) <= 0)  # decreasing row length
+    for L_ in np.unique(lens):  # equal lengths keep their order (stable)
+        rows = lay["n2o"][lens == L_]
+        assert np.all(np.diff(rows) > 0)
+    assert np.array_equal(lay["ncol"], [(np.diff(ptr) > k).sum() for k in range(lens[0])])
+
+
+def test_groundwater_flow_is_the_head_form(ffi):
+    """GROUNDWATER_FLOW: Laplace K*tuf (no viscosity), mass = storage, no
+    gravity term even with a density set, the same parabolic step; its
+    Gauss-point velocity is -K grad h."""
+    rng = np.random.default_rng(4)
+    m = M.hex_mesh(2, 2, 2)
+    T = np.array([[2.0, 0.3, 0.1], [0.2, 1.0, -0.2], [0.1, -0.1, 0.5]]) * 1e-5
+    kw = dict(mass_coef=np.full(m.n_elem, 3e-4), cond=np.tile(T.ravel(), m.n_elem),
+              cond_stride=9, tuf=86400.0, dt=2.0, theta=1.0, u_old=rng.standard_normal(m.n_nodes))
+    gw = ffi.OracleProblem(m, process=2, rho_g=9.81e3, mu=7.0, **kw)
+    liq = ffi.OracleProblem(m, process=LIQ, rho_g=0.0, mu=1.0, **kw)
+    for e in range(m.n_elem):
+        Mg, Lg, _, Gg = gw.element_matrices(e)
+        Ml, Ll, _, _ = liq.element_matrices(e)
+        assert np.array_equal(Mg, Ml) and np.array_equal(Lg, Ll)  # tuf*K/1 == K*tuf*1
+        assert not Gg.any()
+        assert all(np.array_equal(a, b) for a, b in zip(gw.element_system(e),
+                                                        liq.element_system(e)))
+    g = np.array([0.4, -1.3, 0.8])
+    h = np.stack([m.x, m.y, m.z], 1) @ g
+    v = gw.gauss_velocity(h)
+    assert np.abs(v - (-(T @ g))[None, :, None]).max() < 1e-12 * np.abs(T @ g).max()
