@@ -143,7 +143,7 @@ struct ogsx_system {
     int *sell_ptr = nullptr, *sell_w = nullptr, *sell_cb = nullptr, *sell_c32 = nullptr,
         *sell_src = nullptr;
     uint16_t* sell_c16 = nullptr;
-    bool pat_ok = false;  // stencil dictionary built (formats 13, 14)
+    bool pat_ok = false;  // stencil dictionary built (formats 13, 14, 17, 18)
     uint8_t* sell_pid = nullptr;
     int* pat_delta = nullptr;
     int pat_w = 0, pat_cnt = 0;
@@ -251,6 +251,11 @@ int dalloc(ogsx_system* s, T** p, size_t count)
         if (rc_ != OGSX_OK) return rc_; \
     } while (0)
 
+inline bool is_pat(int f)
+{
+    return f == kFmtSellPat8 || f == kFmtSellPat7 || f == kFmtSellPat8C || f == kFmtSellPat7C;
+}
+
 bool valid_fmt(const ogsx_system* s, int f)
 {
     if (f == kFmtTile32 || f == kFmtSell32) return true;
@@ -258,7 +263,7 @@ bool valid_fmt(const ogsx_system* s, int f)
     if (f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT || f == kFmtSell16U16 ||
         f == kFmtSell16X2 || f == kFmtSell16U7)
         return s->sell16;
-    if (f == kFmtSellPat8 || f == kFmtSellPat7) return s->pat_ok;
+    if (is_pat(f)) return s->pat_ok;
     return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
 }
 
@@ -285,8 +290,7 @@ MatView<VT> mat_view(const ogsx_system* s, const VT* val)
 inline bool is_sell(int f)
 {
     return f == kFmtSell32 || f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT ||
-           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7 || f == kFmtSellPat8 ||
-           f == kFmtSellPat7;
+           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7 || is_pat(f);
 }
 
 // SELL-64 layout of the pattern: slice widths, offsets, columns (32-bit and,
@@ -543,6 +547,8 @@ int prof_flush(ogsx_system* s)
         case kFmtSell16U7: { constexpr int L = kFmtSell16U7; CALL; } break; \
         case kFmtSellPat8: { constexpr int L = kFmtSellPat8; CALL; } break; \
         case kFmtSellPat7: { constexpr int L = kFmtSellPat7; CALL; } break; \
+        case kFmtSellPat8C: { constexpr int L = kFmtSellPat8C; CALL; } break; \
+        case kFmtSellPat7C: { constexpr int L = kFmtSellPat7C; CALL; } break; \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -575,7 +581,8 @@ int tune_one(ogsx_system* s, int* fmt_out)
 {
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
                          kFmtSell16X2,  kFmtSell16U7, kFmtSellPat8, kFmtSellPat7,
-                         kFmtSell32,    kFmtTile16,   kFmtTile32};
+                         kFmtSellPat8C, kFmtSellPat7C, kFmtSell32,  kFmtTile16,
+                         kFmtTile32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -626,7 +633,7 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
                       fmt == kFmtSell16U7)
                          ? 2
                          : 4;
-    if (fmt == kFmtSellPat8 || fmt == kFmtSellPat7)  // values, slice table, one id per row
+    if (is_pat(fmt))  // values, slice table, one id per row
         return (double)s->npad * sval + 8.0 * s->n_slices + (double)s->n_slices * kSellC +
                2 * n * sx;
     if (is_sell(fmt))  // stored entries incl. padding, slice table (ptr, width, base)
@@ -1590,7 +1597,7 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
     }
     if (!valid_fmt(s, lanes_per_row))
         return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
-                                 "3/5/6/7/10/11/12 (SELL-64), 13/14 (SELL-64 with a stencil "
+                                 "3/5/6/7/10/11/12 (SELL-64), 13/14/17/18 (SELL-64 with a stencil "
                                  "dictionary), or 4, 8, 16, 32, 64 lanes per row (16-bit and "
                                  "dictionary forms only when the pattern allows)");
     if (needs_extra_layouts(lanes_per_row) && !s->extra_layouts) {

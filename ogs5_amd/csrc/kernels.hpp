@@ -230,6 +230,8 @@ constexpr int kFmtSell16X2 = 11;          // SELL-64, 16-bit, two slices per wav
 constexpr int kFmtSell16U7 = 12;          // SELL-64, 16-bit, 7 entries in flight, NT
 constexpr int kFmtSellPat8 = 13;          // SELL-64, stencil dictionary, 8 in flight, NT
 constexpr int kFmtSellPat7 = 14;          // SELL-64, stencil dictionary, 7 in flight, NT
+constexpr int kFmtSellPat8C = 17;         // ... 8 in flight, cached loads
+constexpr int kFmtSellPat7C = 18;         // ... 7 in flight, cached loads
 constexpr int kSellC = 64;                // rows per slice = wavefront size
 constexpr int kPatLds = 4096;             // dictionary entries staged in LDS
 
@@ -517,6 +519,10 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
         spmv_sell_pat<8, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSellPat7)
         spmv_sell_pat<7, true, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSellPat8C)
+        spmv_sell_pat<8, false, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSellPat7C)
+        spmv_sell_pat<7, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16X2)
         spmv_sell_x2<4, VT, XT, ACC>(m, x, G, b, epi);
     else

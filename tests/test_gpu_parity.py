@@ -101,7 +101,7 @@ def test_assembly_parity(name, bc):
     assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
 
 
-@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18, 4, 8, 16, 32, 64])
 def test_spmv_parity_all_variants(lpr):
     sc = SCEN["hex"]
     A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_ROW)
@@ -111,7 +111,7 @@ def test_spmv_parity_all_variants(lpr):
     assert s.spmv_variant() == lpr
     eq.a[:] = A  # same matrix on both sides
     yr = eq.spmv(x)
-    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14):
+    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18):
         # CRS-order summation over rounded products: the reference's multiVec bits
         assert np.array_equal(y, yr)
         return
@@ -122,7 +122,7 @@ def test_spmv_parity_all_variants(lpr):
 
 
 @pytest.mark.parametrize("name", ["quad", "tet", "hex_aniso_grav", "bmskel"])
-@pytest.mark.parametrize("fmt", [13, 14])
+@pytest.mark.parametrize("fmt", [13, 14, 17, 18])
 def test_spmv_stencil_dictionary_bit_exact(name, fmt):
     """The dictionary formats on 2-D, tet and small meshes (padding rows in
     the last slice): the reference's multiVec bits."""
