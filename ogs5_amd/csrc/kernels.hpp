@@ -1213,33 +1213,49 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
-    double ps[1];
-    reduce_partials<1>(gl(S.part), c.G, ps);
-    const double rr = st->rr;
-    const T alpha = (T)(rr / ps[0]);
     const GVec<T> v = gvec(kv_of<T>(S));
     OGSX_GL T* __restrict__ x = v.x;
     OGSX_GL T* __restrict__ r = v.r;
     const OGSX_GL T* __restrict__ p = v.p;
     const OGSX_GL T* __restrict__ sv = v.s;
     const OGSX_GL T* __restrict__ d = v.d;
-    {
-        const int b = c.b;
-        double acc[2] = {0.0, 0.0};
-        int beg, end;
-        chunk_of(S.n, c.G, b, beg, end);
-        constexpr int U = 4;
-        int i = beg + threadIdx.x;
-        for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
-            T xi[U], pi[U], ri[U], si[U], di[U];
+    const int b = c.b;
+    int beg, end;
+    chunk_of(S.n, c.G, b, beg, end);
+    constexpr int U = 4;
+    int i = beg + threadIdx.x;
+    // the first batch of the stream does not depend on alpha: its loads are
+    // issued before the partials are reduced, so their latency overlaps
+    T xi[U], pi[U], ri[U], si[U], di[U];
+    const bool first = i + (U - 1) * kBlock < end;
+    if (first) {
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int k = i + u * kBlock;
-                xi[u] = x[k];
-                pi[u] = p[k];
-                ri[u] = r[k];
-                si[u] = sv[k];
-                di[u] = d[k];
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            xi[u] = x[k];
+            pi[u] = p[k];
+            ri[u] = r[k];
+            si[u] = sv[k];
+            di[u] = d[k];
+        }
+    }
+    double ps[1];
+    reduce_partials<1>(gl(S.part), c.G, ps);
+    const double rr = st->rr;
+    const T alpha = (T)(rr / ps[0]);
+    {
+        double acc[2] = {0.0, 0.0};
+        for (bool loaded = first; i + (U - 1) * kBlock < end; i += U * kBlock, loaded = false) {
+            if (!loaded) {
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int k = i + u * kBlock;
+                    xi[u] = x[k];
+                    pi[u] = p[k];
+                    ri[u] = r[k];
+                    si[u] = sv[k];
+                    di[u] = d[k];
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -1279,6 +1295,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    OGSX_GL T* __restrict__ p = v.p;
+    const OGSX_GL T* __restrict__ r = v.r;
+    const OGSX_GL T* __restrict__ d = v.d;
+    int beg, end;
+    chunk_of(S.n, c.G, c.b, beg, end);
+    constexpr int U = 4;
+    int i = beg + threadIdx.x;
+    // first batch loaded before the reduction (independent of beta)
+    T pi[U], ri[U], di[U];
+    const bool first = i + (U - 1) * kBlock < end;
+    if (first) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            pi[u] = p[k];
+            ri[u] = r[k];
+            di[u] = d[k];
+        }
+    }
     double s[2];
     reduce_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, s);
     const double rnorm = sqrt(s[0]);
@@ -1293,23 +1329,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
         return;
     }
     const T beta = (T)(s[1] / st->rr_old);
-    const GVec<T> v = gvec(kv_of<T>(S));
-    OGSX_GL T* __restrict__ p = v.p;
-    const OGSX_GL T* __restrict__ r = v.r;
-    const OGSX_GL T* __restrict__ d = v.d;
     {
-        int beg, end;
-        chunk_of(S.n, c.G, c.b, beg, end);
-        constexpr int U = 4;
-        int i = beg + threadIdx.x;
-        for (; i + (U - 1) * kBlock < end; i += U * kBlock) {
-            T pi[U], ri[U], di[U];
+        for (bool loaded = first; i + (U - 1) * kBlock < end; i += U * kBlock, loaded = false) {
+            if (!loaded) {
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int k = i + u * kBlock;
-                pi[u] = p[k];
-                ri[u] = r[k];
-                di[u] = d[k];
+                for (int u = 0; u < U; u++) {
+                    const int k = i + u * kBlock;
+                    pi[u] = p[k];
+                    ri[u] = r[k];
+                    di[u] = d[k];
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
