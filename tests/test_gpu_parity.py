@@ -44,6 +44,17 @@ def _scenarios():
     gw.process, gw.tuf, gw.mu, gw.rho_g = ox.GROUNDWATER_FLOW, 86400.0, 5.0, 9.81e3
     gw.steps = [W.Step(st.dt / 86400.0, st.theta) for st in gw.steps]
     out["quad_gw"] = gw
+    # other $ELE_GAUSS_POINTS: 1, 2 and 4 points per direction (4^3 = 64 on
+    # hexes, the largest Gauss table)
+    for name, sc2, ng in (("quad_ng1", W.quad_flow(10, 8, seed=11, n_steps=2), 1),
+                          ("hex_ng2", W.hex_flow(5, seed=12, n_transient=1), 2),
+                          ("hex_ng4", W.hex_flow(4, seed=13, n_transient=1, steady=False), 4),
+                          ("quad_ng4", W.quad_flow(12, 9, seed=14, n_steps=2), 4)):
+        sc2.n_gauss = ng
+        out[name] = sc2
+    out["hex_ng4"].cond = np.tile((T / 1e-4).ravel(), (out["hex_ng4"].mesh.n_elem, 1)) * \
+        out["hex_ng4"].cond[:, None]
+    out["hex_ng4"].cond_stride = 9
     return out
 
 
@@ -179,7 +190,8 @@ def test_reference_solution_chain_is_exact():
         assert np.linalg.norm(ref["steps"][0] - x) / np.linalg.norm(x) < 1e-12
 
 
-@pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav", "quad_gw"])
+@pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav", "quad_gw", "hex_ng4",
+                                  "quad_ng1"])
 @pytest.mark.parametrize("method,bc,prec", [
     (ox.CG, ox.DIRICHLET_SYMMETRIC, ox.FP64),
     (ox.BICGSTAB, ox.DIRICHLET_ROW, ox.FP64),

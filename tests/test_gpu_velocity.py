@@ -69,6 +69,9 @@ def _flow_pair(m, rng, *, tensor, n_gauss, rho_g, tuf=1.0, mu=1.3e-3, process=No
     ("hex", False, 2, False),
     ("tet", True, 3, True),
     ("tet", False, 3, False),
+    ("hex", True, 4, True),   # 64 points: the kernel stores without the LDS stage
+    ("quad", False, 4, False),
+    ("line", False, 1, False),
 ])
 def test_velocity_matches_oracle(kind, tensor, n_gauss, gravity):
     rng = np.random.default_rng(21)
