@@ -303,18 +303,20 @@ def read_gli(path: str) -> Geometry:
     return g
 
 
+_EDGES = {
+    LINE2: [(0, 1)],
+    QUAD4: [(0, 1), (1, 2), (2, 3), (3, 0)],
+    HEX8: [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4),
+           (0, 4), (1, 5), (2, 6), (3, 7)],
+    TET4: [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)],
+}
+
+
 def _min_edge(mesh: Mesh) -> float:
-    X = mesh.xyz
-    c = mesh.conn
-    npe = c.shape[1]
-    pairs = [(0, 1)] if npe == 2 else [(a, (a + 1) % (4 if mesh.etype != TET4 else npe))
-                                       for a in range(4 if mesh.etype != HEX8 else 4)]
-    if mesh.etype == HEX8:
-        pairs += [(a + 4, (a + 1) % 4 + 4) for a in range(4)] + [(a, a + 4) for a in range(4)]
-    if mesh.etype == TET4:
-        pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
-    d = min(np.linalg.norm(X[c[:, a]] - X[c[:, b]], axis=1).min() for a, b in pairs)
-    return float(d)
+    """Shortest element edge (CFEMesh::getMinEdgeLength)."""
+    X, c = mesh.xyz, mesh.conn
+    return float(min(np.linalg.norm(X[c[:, a]] - X[c[:, b]], axis=1).min()
+                     for a, b in _EDGES[mesh.etype]))
 
 
 def geo_nodes(mesh: Mesh, geo: Geometry, gtype: str, name: Optional[str]) -> np.ndarray:
@@ -341,8 +343,7 @@ def geo_nodes(mesh: Mesh, geo: Geometry, gtype: str, name: Optional[str]) -> np.
             t = np.clip(((X - a) @ ab) / max(ab @ ab, 1e-300), 0.0, 1.0)
             d = np.linalg.norm(X - (a + t[:, None] * ab[None, :]), axis=1)
             hit |= d <= r
-        # ordered by the distance of the projection along the polyline
-        return np.nonzero(hit)[0].astype(np.int64)
+        return np.nonzero(hit)[0].astype(np.int64)  # node order
     raise DeckError(f"geometry type {gtype} is not supported")
 
 
