@@ -119,6 +119,9 @@ int ogsx_pattern_destroy(ogsx_pattern* p);
 /* ---- system: Linear_EQS + the element data it is assembled from -------- */
 /* Linear_EQS::Linear_EQS (FEM/equation_class.cpp:85): device A (CRS values
  * on the pattern), b, x and Krylov buffers; bNorm starts at 1.0. */
+/* Fails with OGSX_EINVAL for an empty pattern or a node that belongs to no
+ * element (its CRS row is empty: the reference would read an undefined
+ * diagonal there). */
 int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out);
 int ogsx_system_destroy(ogsx_system* s);
 /* Device bytes the system has allocated so far. */

@@ -785,6 +785,13 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     if (precision != OGSX_FP64 && precision != OGSX_FP32_IR)
         return fail(OGSX_EINVAL, "unknown precision");
     *out = nullptr;
+    if (p->h.n == 0) return fail(OGSX_EINVAL, "empty pattern (no nodes)");
+    // a node in no element has an empty CRS row and no diagonal, which the
+    // Jacobi preconditioner and Dirichlet elimination read
+    for (int64_t i = 0; i < p->h.n; i++)
+        if (p->h.diag[i] < 0)
+            return fail(OGSX_EINVAL, "node " + std::to_string(i) +
+                                         " belongs to no element (empty matrix row)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(OGSX_EHIP, "no HIP device available");

@@ -63,3 +63,16 @@ def test_system_create_without_gpu_fails_loudly(ox):
     p = ox.Pattern(m.n_nodes, m.etype, m.conn)
     with pytest.raises(ox.OgsxError, match="no HIP device"):
         ox.System(p)
+
+
+def test_system_create_rejects_empty_rows(ox):
+    """A node in no element has an empty CRS row (the oracle's pattern has
+    the same row and no diagonal, diag = -1): refused before any device work,
+    so it needs no GPU."""
+    from oracle import ffi
+    conn = np.array([[0, 1], [1, 2]], np.int64)
+    p = ox.Pattern(5, ox.LINE2, conn)
+    ptr, col, diag = p.export()
+    assert list(diag) == list(ffi.pattern(5, conn)[2]) == [0, 3, 6, -1, -1]
+    with pytest.raises(ox.OgsxError, match="node 3 belongs to no element"):
+        ox.System(p)
