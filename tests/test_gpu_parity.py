@@ -151,8 +151,9 @@ def test_stencil_dictionary_needs_repeated_patterns():
     dictionary: the formats are refused, the others still work."""
     rng = np.random.default_rng(3)
     n = 3000
-    conn = np.stack([np.arange(n), rng.integers(0, n, n)], 1)
-    conn = conn[conn[:, 0] != conn[:, 1]]
+    chain = np.stack([np.arange(n - 1), np.arange(1, n)], 1)  # every node in an element
+    extra = np.stack([np.arange(n), rng.integers(0, n, n)], 1)
+    conn = np.concatenate([chain, extra[extra[:, 0] != extra[:, 1]]])
     pat = ox.Pattern(n, ox.LINE2, conn)
     s = ox.System(pat, ox.FP64)
     with pytest.raises(ox.OgsxError):
