@@ -296,7 +296,8 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
 #define OGSX_PROF_ASSEMBLY 2
 #define OGSX_PROF_VELOCITY 3 /* ogsx_gauss_velocity kernel (on the flow system) */
 /* SpMV storage/launch variant: 0 = autotune (time variants 1, 2, 3, 5, 6, 7,
- * 10, 11, 12, 13, 14, 17, 18 on this matrix at first use and keep the fastest; all of them sum each row in
+ * 10, 11, 12, 13, 14, 17, 18, 19, 20 on this matrix at first use and keep the
+ * fastest; all of them sum each row in
  * CRS order over rounded products, bit-identical to CSparseMatrix::multiVec,
  * so the choice never changes results); 1 = tiled CRS (blocks stream
  * contiguous nonzero tiles through LDS); 2 = tiled CRS with 16-bit column
@@ -310,7 +311,12 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * row, columns = row + the pattern's offsets; when the pattern has at most
  * 255 distinct offset lists), 8 / 7 entries in flight, non-temporal; 17 / 18
  * = as 13 / 14 with cached matrix loads (a matrix that fits the caches stays
- * there between launches); 4, 8, 16, 32,
+ * there between launches); 19 / 20 = SELL-64 with one stencil per slice
+ * in triples of consecutive column offsets (one coalesced x load per triple,
+ * its +1 / +2 neighbours by lane shifts; zero entries where a row lacks a
+ * column of its slice's stencil; when every slice needs at most 16 triples
+ * and the stored entries are at most twice the nonzeros), 5 / 4 triples in
+ * flight, non-temporal / cached; 4, 8, 16, 32,
  * 64 = vector CRS with that many lanes per row (lane-parallel partial sums:
  * a different rounding order).  The CRS arrays stay the system's matrix;
  * SELL copies are refreshed after A changes. */

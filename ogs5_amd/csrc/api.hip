@@ -83,6 +83,12 @@ struct PatternLayout {
     std::vector<int> pdelta;
     int pat_w = 0, pat_cnt = 0;
     bool pat_ok = false;
+    // slice triples (formats 19, 20): per slice its stencil id and first
+    // entry, per stencil its triples, and the CRS position of every entry
+    std::vector<int> tptr, tcnt, toff, tsrc;
+    std::vector<uint16_t> tid;
+    int64_t npad3 = 0;
+    bool tri_ok = false;
 };
 
 struct ogsx_pattern {
@@ -147,9 +153,16 @@ struct ogsx_system {
     uint8_t* sell_pid = nullptr;
     int* pat_delta = nullptr;
     int pat_w = 0, pat_cnt = 0;
+    bool tri_ok = false;  // slice triples built (formats 19, 20)
+    int64_t npad3 = 0;
+    int *tri_ptr = nullptr, *tri_cnt = nullptr, *tri_off = nullptr, *tri_src = nullptr;
+    uint16_t* tri_id = nullptr;
+    // values in SELL-64 (layout 0) or slice-triple (layout 1) order, as the
+    // precision's format reads them
     double* sval64 = nullptr;
     float* sval32 = nullptr;
     uint64_t a_version = 1, sval64_version = 0, sval32_version = 0, a32_version = 0;
+    int sval64_layout = 0, sval32_layout = 0;
     hipStream_t stream = nullptr;
     // pattern on device
     int *ptr = nullptr, *col = nullptr, *diagpos = nullptr;
@@ -256,6 +269,11 @@ inline bool is_pat(int f)
     return f == kFmtSellPat8 || f == kFmtSellPat7 || f == kFmtSellPat8C || f == kFmtSellPat7C;
 }
 
+inline bool is_tri(int f)
+{
+    return f == kFmtTri5 || f == kFmtTri4C;
+}
+
 bool valid_fmt(const ogsx_system* s, int f)
 {
     if (f == kFmtTile32 || f == kFmtSell32) return true;
@@ -264,6 +282,7 @@ bool valid_fmt(const ogsx_system* s, int f)
         f == kFmtSell16X2 || f == kFmtSell16U7)
         return s->sell16;
     if (is_pat(f)) return s->pat_ok;
+    if (is_tri(f)) return s->tri_ok;
     return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
 }
 
@@ -284,13 +303,14 @@ MatView<VT> mat_view(const ogsx_system* s, const VT* val)
     return MatView<VT>{s->n,        s->n_tiles,  s->ptr,      s->col,     s->col16,
                        val,         s->tile_row, s->tile_cb,  s->n_slices, s->sell_ptr,
                        s->sell_w,   s->sell_cb,  s->sell_c16, s->sell_c32, sv,
-                       s->sell_pid, s->pat_delta, s->pat_w,   s->pat_cnt};
+                       s->sell_pid, s->pat_delta, s->pat_w,   s->pat_cnt,  s->tri_ptr,
+                       s->tri_id,   s->tri_cnt,  s->tri_off};
 }
 
 inline bool is_sell(int f)
 {
     return f == kFmtSell32 || f == kFmtSell16 || f == kFmtSell16U8 || f == kFmtSell16NT ||
-           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7 || is_pat(f);
+           f == kFmtSell16U16 || f == kFmtSell16X2 || f == kFmtSell16U7 || is_pat(f) || is_tri(f);
 }
 
 // SELL-64 layout of the pattern: slice widths, offsets, columns (32-bit and,
@@ -395,17 +415,21 @@ void build_pattern_dict(const PatternHost& h, const std::vector<int>& sw, Patter
     L.pat_ok = true;
 }
 
-// Brings the SELL value copies up to date with A (after assembly / BC).
+// Brings the SELL value copies up to date with A (after assembly / BC), in
+// the layout of the precision's format (SELL-64 or slice triples).
 int sync_sell(ogsx_system* s, bool want64, bool want32)
 {
     const Grp g1{s->dev, nullptr, nullptr};
-    if (want64 && s->sval64_version != s->a_version) {
-        k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1);
+    const int lay64 = is_tri(s->fmt64) ? 1 : 0, lay32 = is_tri(s->fmt32) ? 1 : 0;
+    if (want64 && (s->sval64_version != s->a_version || s->sval64_layout != lay64)) {
+        k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1, lay64);
         s->sval64_version = s->a_version;
+        s->sval64_layout = lay64;
     }
-    if (want32 && s->sval32 && s->sval32_version != s->a_version) {
-        k_to_sell_g<float><<<s->G, kBlock, 0, s->stream>>>(g1);
+    if (want32 && s->sval32 && (s->sval32_version != s->a_version || s->sval32_layout != lay32)) {
+        k_to_sell_g<float><<<s->G, kBlock, 0, s->stream>>>(g1, lay32);
         s->sval32_version = s->a_version;
+        s->sval32_layout = lay32;
     }
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
@@ -449,6 +473,82 @@ void build_tiles(const PatternHost& h, std::vector<int>& trow, std::vector<int>&
                 c16[k] = (uint16_t)(h.col[k] - tcb[t]);
 }
 
+// Slice triples: each slice's stencil is the sorted union of its rows'
+// column offsets, cut greedily into triples o, o+1, o+2 (an offset the
+// union lacks becomes a zero entry).  Built when every slice needs at most
+// kTriMax triples, there are at most 65535 distinct slice stencils, and the
+// stored entries are at most twice the nonzeros (grid-numbered meshes: 1.01
+// on the 128^3 hexes; small meshes pay more at their boundaries); otherwise
+// the formats are not offered.
+void build_triples(const PatternHost& h, PatternLayout& L)
+{
+    const int64_t ns = (h.n + kSellC - 1) / kSellC;
+    if (h.n == 0) return;
+    std::map<std::vector<int>, int> ids;
+    std::vector<std::vector<int>> stencils;
+    std::vector<uint16_t> tid(ns);
+    std::vector<int> tptr(ns + 1, 0);
+    std::vector<int> u, starts;
+    int64_t total = 0;
+    for (int64_t sl = 0; sl < ns; sl++) {
+        u.clear();
+        const int64_t r1 = std::min<int64_t>(h.n, (sl + 1) * kSellC);
+        for (int64_t r = sl * kSellC; r < r1; r++)
+            for (int64_t k = h.ptr[r]; k < h.ptr[r + 1]; k++) u.push_back((int)(h.col[k] - r));
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if (u.empty()) return;
+        starts.clear();
+        for (size_t i = 0; i < u.size();) {
+            const int o = u[i];
+            starts.push_back(o);
+            while (i < u.size() && u[i] <= o + 2) i++;
+        }
+        if ((int)starts.size() > kTriMax) return;
+        auto it = ids.find(starts);
+        int id;
+        if (it == ids.end()) {
+            if (stencils.size() >= 65535) return;
+            id = (int)stencils.size();
+            ids.emplace(starts, id);
+            stencils.push_back(starts);
+        } else {
+            id = it->second;
+        }
+        tid[sl] = (uint16_t)id;
+        total += 3 * kSellC * (int64_t)starts.size();
+        if (total > 2 * h.nnz + 3 * kSellC || total >= (int64_t(1) << 31)) return;
+        tptr[sl + 1] = (int)total;
+    }
+    L.tsrc.assign(total, -1);
+    for (int64_t sl = 0; sl < ns; sl++) {
+        const std::vector<int>& st = stencils[tid[sl]];
+        const int64_t r1 = std::min<int64_t>(h.n, (sl + 1) * kSellC);
+        for (int64_t r = sl * kSellC; r < r1; r++) {
+            const int l = (int)(r - sl * kSellC);
+            const int64_t* c0 = h.col.data() + h.ptr[r];
+            const int64_t* c1 = h.col.data() + h.ptr[r + 1];
+            for (size_t k = 0; k < st.size(); k++)
+                for (int t = 0; t < 3; t++) {
+                    const int64_t c = r + st[k] + t;
+                    const int64_t* f = std::lower_bound(c0, c1, c);
+                    if (f != c1 && *f == c)
+                        L.tsrc[tptr[sl] + kSellC * (3 * (int64_t)k + t) + l] = (int)(f - h.col.data());
+                }
+        }
+    }
+    L.tcnt.assign(stencils.size(), 0);
+    L.toff.assign(stencils.size() * kTriMax, 0);
+    for (size_t q = 0; q < stencils.size(); q++) {
+        L.tcnt[q] = (int)stencils[q].size();
+        for (size_t k = 0; k < stencils[q].size(); k++) L.toff[q * kTriMax + k] = stencils[q][k];
+    }
+    L.tid.swap(tid);
+    L.tptr.swap(tptr);
+    L.npad3 = total;
+    L.tri_ok = true;
+}
+
 void build_layout(const PatternHost& h, PatternLayout& L)
 {
     L.ptr32.resize(h.n + 1);
@@ -462,6 +562,7 @@ void build_layout(const PatternHost& h, PatternLayout& L)
     build_sell(h, L.sptr, L.sw, L.scb, L.sc32, L.sc16, L.ssrc, L.sell16);
     L.npad = L.sptr.back();
     build_pattern_dict(h, L.sw, L);
+    build_triples(h, L);
 }
 
 // Device bytes a system on pattern p allocates: ogsx_system_create plus
@@ -486,6 +587,12 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
          b((int64_t)L.ssrc.size(), 4) + b(L.npad, 8) +
          (precision == OGSX_FP32_IR ? b(L.npad, 4) : 0) + (L.sell16 ? b((int64_t)L.sc16.size(), 2) : 0);
     if (L.pat_ok) t += b((int64_t)L.spid.size(), 1) + b((int64_t)L.pdelta.size(), 4);
+    if (L.tri_ok) {  // tables, entry map, value arrays sized for the longer layout
+        const int64_t extra = std::max<int64_t>(0, L.npad3 - L.npad);
+        t += b((int64_t)L.tptr.size(), 4) + b((int64_t)L.tid.size(), 2) + b((int64_t)L.tcnt.size(), 4) +
+             b((int64_t)L.toff.size(), 4) + b(L.npad3, 4) + extra * 8 +
+             (precision == OGSX_FP32_IR ? extra * 4 : 0);
+    }
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
          b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
@@ -549,6 +656,8 @@ int prof_flush(ogsx_system* s)
         case kFmtSellPat7: { constexpr int L = kFmtSellPat7; CALL; } break; \
         case kFmtSellPat8C: { constexpr int L = kFmtSellPat8C; CALL; } break; \
         case kFmtSellPat7C: { constexpr int L = kFmtSellPat7C; CALL; } break; \
+        case kFmtTri5: { constexpr int L = kFmtTri5; CALL; } break;           \
+        case kFmtTri4C: { constexpr int L = kFmtTri4C; CALL; } break;         \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -581,8 +690,8 @@ int tune_one(ogsx_system* s, int* fmt_out)
 {
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
                          kFmtSell16X2,  kFmtSell16U7, kFmtSellPat8, kFmtSellPat7,
-                         kFmtSellPat8C, kFmtSellPat7C, kFmtSell32,  kFmtTile16,
-                         kFmtTile32};
+                         kFmtSellPat8C, kFmtSellPat7C, kFmtTri5,    kFmtTri4C,
+                         kFmtSell32,    kFmtTile16,   kFmtTile32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -633,6 +742,8 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
                       fmt == kFmtSell16U7)
                          ? 2
                          : 4;
+    if (is_tri(fmt))  // values incl. zero entries, slice table (first entry, stencil id)
+        return (double)s->npad3 * sval + 6.0 * s->n_slices + 2 * n * sx;
     if (is_pat(fmt))  // values, slice table, one id per row
         return (double)s->npad * sval + 8.0 * s->n_slices + (double)s->n_slices * kSellC +
                2 * n * sx;
@@ -858,12 +969,14 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     s->sell16 = Ly.sell16;
     s->n_slices = (int)Ly.sw.size();
     s->npad = Ly.npad;
+    s->npad3 = Ly.tri_ok ? Ly.npad3 : 0;
     {
+        const int64_t nval = std::max(s->npad, s->npad3);
         if ((rc = dalloc(s, &s->sell_ptr, Ly.sptr.size())) || (rc = dalloc(s, &s->sell_w, Ly.sw.size())) ||
             (rc = dalloc(s, &s->sell_cb, Ly.scb.size())) ||
-            (rc = dalloc(s, &s->sell_src, Ly.ssrc.size())) || (rc = dalloc(s, &s->sval64, s->npad)))
+            (rc = dalloc(s, &s->sell_src, Ly.ssrc.size())) || (rc = dalloc(s, &s->sval64, nval)))
             return cleanup(rc);
-        if (precision == OGSX_FP32_IR && (rc = dalloc(s, &s->sval32, s->npad))) return cleanup(rc);
+        if (precision == OGSX_FP32_IR && (rc = dalloc(s, &s->sval32, nval))) return cleanup(rc);
         if (s->sell16 && (rc = dalloc(s, &s->sell_c16, Ly.sc16.size()))) return cleanup(rc);
         if (up(s->sell_ptr, Ly.sptr.data(), 4 * Ly.sptr.size()) || up(s->sell_w, Ly.sw.data(), 4 * Ly.sw.size()) ||
             up(s->sell_cb, Ly.scb.data(), 4 * Ly.scb.size()) ||
@@ -880,6 +993,19 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
             s->pat_ok = true;
             s->pat_w = Ly.pat_w;
             s->pat_cnt = Ly.pat_cnt;
+        }
+        if (Ly.tri_ok) {
+            if ((rc = dalloc(s, &s->tri_ptr, Ly.tptr.size())) || (rc = dalloc(s, &s->tri_id, Ly.tid.size())) ||
+                (rc = dalloc(s, &s->tri_cnt, Ly.tcnt.size())) || (rc = dalloc(s, &s->tri_off, Ly.toff.size())) ||
+                (rc = dalloc(s, &s->tri_src, Ly.tsrc.size())))
+                return cleanup(rc);
+            if (up(s->tri_ptr, Ly.tptr.data(), 4 * Ly.tptr.size()) ||
+                up(s->tri_id, Ly.tid.data(), 2 * Ly.tid.size()) ||
+                up(s->tri_cnt, Ly.tcnt.data(), 4 * Ly.tcnt.size()) ||
+                up(s->tri_off, Ly.toff.data(), 4 * Ly.toff.size()) ||
+                up(s->tri_src, Ly.tsrc.data(), 4 * Ly.tsrc.size()))
+                return cleanup(fail(OGSX_EHIP, "upload of the slice triples failed"));
+            s->tri_ok = true;
         }
     }
     // the 32-bit-column SELL and tiled layouts are tuner candidates only when
@@ -899,6 +1025,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
         d.sval32 = s->sval32;
         d.diagpos = s->diagpos;
         d.sell_src = s->sell_src;
+        d.tri_src = s->tri_src;
         d.r64 = s->r64;
         d.part = s->part;
         d.red = s->red;
@@ -909,6 +1036,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
         d.G = s->G;
         d.nnz = (int)s->nnz;
         d.npad = (int)s->npad;
+        d.npad3 = (int)s->npad3;
         // no solve yet: the state reads "stopped" so group launches skip it
         KrylovState st0;
         std::memset(&st0, 0, sizeof(st0));
@@ -1605,8 +1733,9 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
     if (!valid_fmt(s, lanes_per_row))
         return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
                                  "3/5/6/7/10/11/12 (SELL-64), 13/14/17/18 (SELL-64 with a stencil "
-                                 "dictionary), or 4, 8, 16, 32, 64 lanes per row (16-bit and "
-                                 "dictionary forms only when the pattern allows)");
+                                 "dictionary), 19/20 (SELL-64 slice triples), or 4, 8, 16, 32, 64 "
+                                 "lanes per row (16-bit, dictionary and triple forms only when the "
+                                 "pattern allows)");
     if (needs_extra_layouts(lanes_per_row) && !s->extra_layouts) {
         OK_TRY(alloc_extra_layouts(s));
         // the matrix views of the device descriptor now hold the new arrays

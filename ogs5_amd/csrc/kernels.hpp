@@ -181,6 +181,16 @@ struct MatView {
     const int* pat_delta;
     int pat_w;
     int pat_cnt;
+    // slice triples: slice s holds rows [64 s, 64 s + 64); its stencil is
+    // tri_cnt[q] triples (q = tri_id[s]) of column offsets o, o+1, o+2 with
+    // o = tri_off[q * kTriMax + k]; entry 3 k + t of lane l (column
+    // 64 s + l + o + t) at tri_ptr[s] + 64 (3 k + t) + l, value 0 where the
+    // row has no such column.  Values live in sval (the layout of the
+    // system's current format).
+    const int* tri_ptr;
+    const uint16_t* tri_id;
+    const int* tri_cnt;
+    const int* tri_off;
 };
 
 // MatView with global-address-space pointers, as the device loops use it.
@@ -205,6 +215,10 @@ struct GMat {
     const OGSX_GL int* pat_delta;
     int pat_w;
     int pat_cnt;
+    const OGSX_GL int* tri_ptr;
+    const OGSX_GL uint16_t* tri_id;
+    const OGSX_GL int* tri_cnt;
+    const OGSX_GL int* tri_off;
 };
 
 template <typename VT>
@@ -214,7 +228,8 @@ __device__ __forceinline__ GMat<VT> gmat(const MatView<VT>& m)
                     gl(m.col16),  gl(m.val),       gl(m.tile_row), gl(m.tile_cb),
                     m.n_slices,   gl(m.sell_ptr),  gl(m.sell_w),   gl(m.sell_cb),
                     gl(m.sell_c16), gl(m.sell_c32), gl(m.sval),    gl(m.sell_pid),
-                    gl(m.pat_delta), m.pat_w,       m.pat_cnt};
+                    gl(m.pat_delta), m.pat_w,       m.pat_cnt,     gl(m.tri_ptr),
+                    gl(m.tri_id),   gl(m.tri_cnt),  gl(m.tri_off)};
 }
 
 constexpr int kTileU = 8;                 // nonzeros per thread per tile
@@ -234,6 +249,9 @@ constexpr int kFmtSellPat8C = 17;         // ... 8 in flight, cached loads
 constexpr int kFmtSellPat7C = 18;         // ... 7 in flight, cached loads
 constexpr int kSellC = 64;                // rows per slice = wavefront size
 constexpr int kPatLds = 4096;             // dictionary entries staged in LDS
+constexpr int kFmtTri5 = 19;              // SELL-64 slice triples, 5 in flight, NT
+constexpr int kFmtTri4C = 20;             // ... 4 in flight, cached loads
+constexpr int kTriMax = 16;               // triples per slice stencil
 
 // Tiled CRS SpMV: a block streams the contiguous nonzeros of one tile of
 // rows (<= kTileNnz nonzeros, <= kBlock rows) with kTileU independent
@@ -432,6 +450,99 @@ __device__ __forceinline__ void spmv_sell_pat(const GMat<VT>& m, const OGSX_GL X
     }
 }
 
+// Lane l of the wave takes lane l + 1's value; lane 63 takes `edge` (DPP
+// wave_shl:1, lanes whose source is past the wave keep the old value).
+__device__ __forceinline__ float wave_next(float v, float edge)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_next(double v, double edge)
+{
+    const long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
+    const int lo = __builtin_amdgcn_update_dpp((int)eb, (int)vb, 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(eb >> 32), (int)(vb >> 32), 0x130, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// SELL-64 with one stencil per slice, in triples of consecutive column
+// offsets: for a mesh numbered along grid lines, the 64 rows of a slice and
+// their neighbours o, o+1, o+2 read x[r0 + l + o ...]: one coalesced load of
+// x[r0 + l + o] per triple, the +1 and +2 neighbours by shifting it one lane
+// across the wave (the wave's last lanes take x[r0 + 64 + o], x[r0 + 65 + o]
+// from scalar loads).  A third of the gathers of the per-row formats, and
+// none of their address arithmetic.  The slice stencil is the union of its
+// rows' offsets, so a row may hold zero entries for columns it lacks; a zero
+// product adds nothing (x finite), the others are summed left to right over
+// rounded products in column order: bit-identical to the other formats.
+// Gathers of slices near the first or last row are clamped to [0, n).
+template <int UT, bool NT, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_tri(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                         int G, int b, Epi&& epi)
+{
+#pragma clang fp contract(off)
+    int sb, se;
+    chunk_of(m.n_slices, G, b, sb, se);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nm1 = m.n - 1;
+    for (int sl0 = sb + wave; sl0 < se; sl0 += kBlock / 64) {
+        const int sl = __builtin_amdgcn_readfirstlane(sl0);  // wave-uniform
+        const int r0 = sl * kSellC;
+        const int q = m.tri_id[sl];
+        const int K = m.tri_cnt[q];
+        const OGSX_GL int* of = m.tri_off + q * kTriMax;
+        const OGSX_GL VT* vb = m.sval + m.tri_ptr[sl];
+        const bool inside = r0 + of[0] >= 0 && r0 + of[K - 1] + kSellC + 1 <= nm1;
+        ACC s = 0;
+        int k = 0;
+        for (; k + UT <= K; k += UT) {
+            VT v[3 * UT];
+            XT xb[UT], e1[UT], e2[UT];
+#pragma unroll
+            for (int u = 0; u < UT; u++) {
+                const int o = of[k + u];
+#pragma unroll
+                for (int t = 0; t < 3; t++) v[3 * u + t] = ld<NT>(vb + (3 * (k + u) + t) * kSellC + lane);
+                if (inside) {
+                    const OGSX_GL XT* xo = x + (r0 + o);
+                    xb[u] = xo[lane];
+                    e1[u] = xo[kSellC];
+                    e2[u] = xo[kSellC + 1];
+                } else {
+                    xb[u] = x[min(max(r0 + lane + o, 0), nm1)];
+                    e1[u] = x[min(max(r0 + kSellC + o, 0), nm1)];
+                    e2[u] = x[min(max(r0 + kSellC + 1 + o, 0), nm1)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UT; u++) {
+                const XT x1 = wave_next(xb[u], e1[u]);
+                const XT x2 = wave_next(x1, e2[u]);
+                ACC pr = (ACC)v[3 * u] * (ACC)xb[u];
+                s = s + pr;
+                pr = (ACC)v[3 * u + 1] * (ACC)x1;
+                s = s + pr;
+                pr = (ACC)v[3 * u + 2] * (ACC)x2;
+                s = s + pr;
+            }
+        }
+        for (; k < K; k++) {
+            const int o = of[k];
+            const XT x0 = x[min(max(r0 + lane + o, 0), nm1)];
+            const XT x1 = wave_next(x0, x[min(max(r0 + kSellC + o, 0), nm1)]);
+            const XT x2 = wave_next(x1, x[min(max(r0 + kSellC + 1 + o, 0), nm1)]);
+            ACC pr = (ACC)vb[(3 * k) * kSellC + lane] * (ACC)x0;
+            s = s + pr;
+            pr = (ACC)vb[(3 * k + 1) * kSellC + lane] * (ACC)x1;
+            s = s + pr;
+            pr = (ACC)vb[(3 * k + 2) * kSellC + lane] * (ACC)x2;
+            s = s + pr;
+        }
+        const int row = r0 + lane;
+        if (row < m.n) epi(row, s);
+    }
+}
+
 // Two slices per wavefront in one loop: twice the independent loads in
 // flight; each row is still summed in CRS order.
 template <int U, typename VT, typename XT, typename ACC, typename Epi>
@@ -525,6 +636,10 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
         spmv_sell_pat<7, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16X2)
         spmv_sell_x2<4, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtTri5)
+        spmv_tri<5, true, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtTri4C)
+        spmv_tri<4, false, VT, XT, ACC>(m, x, G, b, epi);
     else
         spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, G, b, epi);
 }
@@ -1010,13 +1125,14 @@ struct SysDev {
     float* sval32;
     const int* diagpos;
     const int* sell_src;
+    const int* tri_src;  // CRS position of every slice-triple entry (-1 = zero)
     double* r64;         // IR residual
     double* part;        // partial sums [16][G]
     double* red;         // reduced scalars
     KrylovState* st;
     KrylovParams* prm;
     IRState* irs;        // IR systems
-    int n, G, nnz, npad;
+    int n, G, nnz, npad, npad3;
 };
 
 template <typename VT>
@@ -1087,9 +1203,10 @@ __global__ __launch_bounds__(kBlock) void k_diag(Grp g)
     });
 }
 
-// CRS values -> SELL-64 values (padding 0) of one precision.
+// CRS values -> SELL-64 values (padding 0) of one precision, in the SELL
+// layout (tri = 0) or the slice-triple layout (tri = 1).
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g)
+__global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g, int tri)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
@@ -1098,9 +1215,9 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g)
         sv = gl(S.sval64);
     else
         sv = gl(S.sval32);
-    const OGSX_GL int* src = gl(S.sell_src);
+    const OGSX_GL int* src = gl(tri ? S.tri_src : S.sell_src);
     const OGSX_GL double* A = gl(S.A);
-    vec_vblocks(c, S.npad, [&](int p) {
+    vec_vblocks(c, tri ? S.npad3 : S.npad, [&](int p) {
         const int k = src[p];
         sv[p] = k >= 0 ? (VT)A[k] : (VT)0;
     });

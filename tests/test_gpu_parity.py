@@ -112,7 +112,8 @@ def test_assembly_parity(name, bc):
     assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
 
 
-@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18, 19, 20,
+                                 4, 8, 16, 32, 64])
 def test_spmv_parity_all_variants(lpr):
     sc = SCEN["hex"]
     A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_ROW)
@@ -122,7 +123,7 @@ def test_spmv_parity_all_variants(lpr):
     assert s.spmv_variant() == lpr
     eq.a[:] = A  # same matrix on both sides
     yr = eq.spmv(x)
-    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18):
+    if lpr in (1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18, 19, 20):
         # CRS-order summation over rounded products: the reference's multiVec bits
         assert np.array_equal(y, yr)
         return
@@ -146,6 +147,46 @@ def test_spmv_stencil_dictionary_bit_exact(name, fmt):
     assert np.array_equal(y, eq.spmv(x))
 
 
+@pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav", "bmskel", "quad_ng4"])
+@pytest.mark.parametrize("fmt", [19, 20])
+def test_spmv_slice_triples_bit_exact(name, fmt):
+    """Slice triples (one stencil per 64-row slice, lane-shifted gathers,
+    zero entries where a row lacks a column of its slice's stencil) on 2-D,
+    3-D and 1-D meshes, including a last slice with padding rows and slices
+    whose gathers are clamped at the first and last rows: the reference's
+    multiVec bits."""
+    sc = SCEN[name]
+    A, b, Ar, br, ptr, s, eq = _assemble_both(sc, ox.DIRICHLET_SYMMETRIC)
+    s.set_spmv_variant(fmt)
+    eq.a[:] = A
+    for seed in range(3):
+        x = np.random.default_rng(100 * fmt + seed).standard_normal(sc.n)
+        assert np.array_equal(s.spmv(x), eq.spmv(x))
+    # A changed: the value copy follows it in the triple layout
+    s.set_spmv_variant(13 if fmt == 19 else 5)
+    x = np.random.default_rng(7).standard_normal(sc.n)
+    y13 = s.spmv(x)
+    s.set_spmv_variant(fmt)
+    assert np.array_equal(s.spmv(x), y13)
+
+
+@pytest.mark.parametrize("prec", [ox.FP64, ox.FP32_IR])
+@pytest.mark.parametrize("fmt", [19, 20])
+def test_solve_with_slice_triples(prec, fmt):
+    """CG solves with the triple formats forced (both precisions of an IR
+    system use them) reach the oracle's solution chain."""
+    sc = SCEN["hex"]
+    ref = _tight_reference(sc)
+    pat, s = W.make_system(sc, prec)
+    s.set_spmv_variant(fmt)
+    res = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=prec,
+                    tol=1e-13, keep_steps=True, system=(pat, s))
+    assert s.spmv_variant() == fmt
+    for x, xr in zip(res["steps"], ref["steps"]):
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < SOL_TOL
+    s.close()
+
+
 def test_stencil_dictionary_needs_repeated_patterns():
     """A pattern with more than 255 distinct column-offset lists has no
     dictionary: the formats are refused, the others still work."""
@@ -158,6 +199,8 @@ def test_stencil_dictionary_needs_repeated_patterns():
     s = ox.System(pat, ox.FP64)
     with pytest.raises(ox.OgsxError):
         s.set_spmv_variant(13)
+    with pytest.raises(ox.OgsxError):
+        s.set_spmv_variant(19)  # slice stencils far beyond 16 triples
     s.set_spmv_variant(12)
     s.close()
 
