@@ -322,6 +322,14 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * SELL copies are refreshed after A changes. */
 int ogsx_set_spmv_variant(ogsx_system* s, int variant);
 int ogsx_get_spmv_variant(ogsx_system* s, int* variant);
+/* The SpMV formats in use for the FP64 and (IR systems, else 0) FP32
+ * matrix: the tuner's fastest, a format forced by ogsx_set_spmv_variant, or
+ * one shared with the other systems of a group (ogsx_group_start lets a
+ * starting system take a format timed within 10 % or 4 us of its fastest,
+ * whichever is larger, when that saves the group a distinct format: one
+ * SpMV launch per format and Krylov phase).  No counterpart in the
+ * reference (one matrix format there). */
+int ogsx_get_spmv_formats(ogsx_system* s, int* fmt64, int* fmt32);
 /* The autotuner's table: variant and average ms per SpMV launch (FP64 then,
  * for IR systems, FP32 entries); *count = entries available. */
 int ogsx_spmv_tuning(ogsx_system* s, int max_entries, int* variants, double* ms,

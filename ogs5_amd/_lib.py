@@ -87,6 +87,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_double)]),
     "ogsx_set_spmv_variant": (C.c_int, [_P, C.c_int]),
     "ogsx_get_spmv_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "ogsx_get_spmv_formats": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ogsx_spmv_tuning": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(C.c_int)]),
     "ogsx_bench_spmv": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
@@ -330,12 +331,12 @@ class System:
         return v.value
 
     def spmv_format(self, fp32=False) -> int:
-        """Tuned SpMV format of the FP64 (or, for IR systems, FP32) matrix:
-        the fastest entry of that half of the autotuner's table."""
-        t = self.spmv_tuning()
-        half = len(t) // 2 if self.precision == FP32_IR else len(t)
-        part = t[half:] if fp32 else t[:half]
-        return min(part, key=lambda e: e[1])[0]
+        """SpMV format in use for the FP64 (or, for IR systems, FP32) matrix:
+        the tuner's fastest, a forced variant, or one shared in a group."""
+        f64, f32 = C.c_int(), C.c_int()
+        check(lib().ogsx_get_spmv_formats(self.h, C.byref(f64), C.byref(f32)),
+              "ogsx_get_spmv_formats")
+        return f32.value if fp32 else f64.value
 
     def bench_spmv(self, fp32=False, reps=20):
         ms, b = C.c_double(), C.c_double()

@@ -142,6 +142,8 @@ struct ogsx_system {
     bool extra_layouts = false;  // tiles and 32-bit SELL columns on the device
     const PatternLayout* lay = nullptr;
     std::vector<std::pair<int, double>> tune_log;  // (format, ms) of the last tuning
+    size_t tune_n64 = 0;      // leading FP64 entries of tune_log (then FP32)
+    bool fmt_forced = false;  // formats set by ogsx_set_spmv_variant
     // SELL-64 copy of the matrix (values refreshed from A when A changed)
     int n_slices = 0;
     int64_t npad = 0;
@@ -729,6 +731,7 @@ int autotune(ogsx_system* s)
     if (s->fmt64 == 0) {
         s->tune_log.clear();
         OK_TRY((tune_one<double, double>(s, &s->fmt64)));
+        s->tune_n64 = s->tune_log.size();
     }
     if (s->precision == OGSX_FP32_IR && s->fmt32 == 0) OK_TRY((tune_one<float, float>(s, &s->fmt32)));
     return OGSX_OK;
@@ -1485,6 +1488,89 @@ int member_index(const ogsx_group* g, const ogsx_system* s)
         if (mem[i] == s) return (int)i;
     return -1;
 }
+
+// A starting system may take a format slower (its own tuner timings) than
+// its fastest by this fraction, or by this many ms per launch — about what
+// one more launch per Krylov phase costs the group — whichever is larger,
+// to share a format with the others.
+constexpr double kGroupFmtSlack = 0.10;
+constexpr double kGroupFmtSlackMs = 0.004;
+
+// Formats of systems that start together in a group: every distinct format
+// among the running systems costs one more SpMV launch per Krylov phase
+// (measured on 256 2-D instances: one common format runs 11-14 % faster
+// than each instance's own fastest).  A starting system takes a format the
+// running systems already use, or the one the most starters can share,
+// among the formats its tuning timed within the slack of its fastest.
+// Formats set by ogsx_set_spmv_variant stay.  Results do not depend on the
+// format (all candidates sum rows in CRS order).
+void share_formats(MemberSet* m, const std::vector<int>& which, bool ir)
+{
+    for (int slot = 0; slot < (ir ? 2 : 1); slot++) {
+        auto fmt = [&](ogsx_system* s) -> int& { return slot == 0 ? s->fmt64 : s->fmt32; };
+        std::vector<int> inuse;
+        for (ogsx_system* s : m->mem)
+            if (s->sc.active) inuse.push_back(fmt(s));
+        struct Starter {
+            ogsx_system* s;
+            std::vector<std::pair<double, int>> ok;  // (ms, format), fastest first
+        };
+        std::vector<Starter> todo;
+        for (int i : which) {
+            ogsx_system* s = m->mem[i];
+            const size_t b = slot == 0 ? 0 : s->tune_n64;
+            const size_t e = slot == 0 ? s->tune_n64 : s->tune_log.size();
+            if (s->fmt_forced || b >= e) {
+                inuse.push_back(fmt(s));
+                continue;
+            }
+            double best = 1e300;
+            for (size_t k = b; k < e; k++) best = std::min(best, s->tune_log[k].second);
+            Starter st{s, {}};
+            for (size_t k = b; k < e; k++)
+                if (s->tune_log[k].second - best <= std::max(kGroupFmtSlack * best, kGroupFmtSlackMs))
+                    st.ok.push_back({s->tune_log[k].second, s->tune_log[k].first});
+            std::sort(st.ok.begin(), st.ok.end());
+            todo.push_back(st);
+        }
+        auto in_use = [&](int f) { return std::find(inuse.begin(), inuse.end(), f) != inuse.end(); };
+        std::vector<char> done(todo.size(), 0);
+        size_t left = todo.size();
+        while (left > 0) {
+            // a format already in use, if acceptable
+            for (size_t t = 0; t < todo.size(); t++) {
+                if (done[t]) continue;
+                for (const auto& c : todo[t].ok)
+                    if (in_use(c.second)) {
+                        fmt(todo[t].s) = c.second;
+                        done[t] = 1;
+                        left--;
+                        break;
+                    }
+            }
+            if (left == 0) break;
+            // else the format acceptable to the most remaining starters
+            std::map<int, std::pair<int, double>> votes;  // format -> (count, summed ms)
+            for (size_t t = 0; t < todo.size(); t++)
+                if (!done[t])
+                    for (const auto& c : todo[t].ok) {
+                        auto& v = votes[c.second];
+                        v.first++;
+                        v.second += c.first;
+                    }
+            int pick = 0, cnt = -1;
+            double ms = 0.0;
+            for (const auto& v : votes)
+                if (v.second.first > cnt || (v.second.first == cnt && v.second.second < ms)) {
+                    pick = v.first;
+                    cnt = v.second.first;
+                    ms = v.second.second;
+                }
+            if (cnt <= 0) break;
+            inuse.push_back(pick);
+        }
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -1573,6 +1659,8 @@ int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
             return fail(OGSX_EINVAL, "system listed twice");
         which.push_back(i);
     }
+    for (int i : which) OK_TRY(autotune(m->mem[i]));
+    share_formats(m, which, g->precision == OGSX_FP32_IR);
     for (int i : which) {
         ogsx_system* s = m->mem[i];
         OK_TRY(prepare_member(s, cfg, s->stream));
@@ -1728,6 +1816,7 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
     if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (lanes_per_row == 0) {
         s->fmt64 = s->fmt32 = 0;  // autotune on next use
+        s->fmt_forced = false;
         return OGSX_OK;
     }
     if (!valid_fmt(s, lanes_per_row))
@@ -1748,6 +1837,7 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
         if (s->group) OK_TRY(set_members(s->group->set, s->group->set->mem));
     }
     s->fmt64 = s->fmt32 = lanes_per_row;
+    s->fmt_forced = true;
     return OGSX_OK;
 }
 
@@ -1756,6 +1846,15 @@ int ogsx_get_spmv_variant(ogsx_system* s, int* lanes_per_row)
     if (!s || !lanes_per_row) return fail(OGSX_EINVAL, "null argument");
     OK_TRY(autotune(s));
     *lanes_per_row = s->fmt64;
+    return OGSX_OK;
+}
+
+int ogsx_get_spmv_formats(ogsx_system* s, int* fmt64, int* fmt32)
+{
+    if (!s || !fmt64 || !fmt32) return fail(OGSX_EINVAL, "null argument");
+    OK_TRY(autotune(s));
+    *fmt64 = s->fmt64;
+    *fmt32 = s->precision == OGSX_FP32_IR ? s->fmt32 : 0;
     return OGSX_OK;
 }
 

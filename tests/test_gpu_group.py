@@ -48,6 +48,39 @@ def test_group_bit_identical_to_alone(prec, graphs):
 
 
 @pytest.mark.parametrize("prec", [ox.FP64, ox.FP32_IR])
+def test_group_shares_formats_within_slack(prec):
+    """Systems starting together take a shared SpMV format when their own
+    tuning timed it within 10 % or 4 us of their fastest (the larger); a
+    forced format stays.
+    Results are those of each system alone (every format sums rows in CRS
+    order)."""
+    systems = [W.make_system(sc, prec) for sc in MIX]
+    systems[1][1].set_spmv_variant(5)  # forced: must stay
+    alone = [W.run_gpu(sc, precision=prec, keep_steps=True) for sc in MIX]
+    mix = W.run_group(MIX, precision=prec, keep_steps=True, systems=systems)
+    for k, a in enumerate(alone):
+        _same(a, dict(info=mix["info"][k], steps=mix["steps"][k]))
+    used = set()
+    for k, (_, s) in enumerate(systems):
+        tune = s.spmv_tuning()
+        half = len(tune) // 2 if prec == ox.FP32_IR else len(tune)
+        for fp32, part in ((False, tune[:half]), (True, tune[half:])):
+            if fp32 and prec != ox.FP32_IR:
+                continue
+            f = s.spmv_format(fp32)
+            used.add((fp32, f))
+            if k == 1:
+                assert f == 5
+                continue
+            best = min(ms for _, ms in part)
+            ms_f = dict(part)[f]
+            assert ms_f - best <= max(0.10 * best, 0.004) * (1 + 1e-9), (k, fp32, f, ms_f, best)
+    print(f"[group] prec={prec} formats in use {sorted(used)}", flush=True)
+    for _, s in systems:
+        s.close()
+
+
+@pytest.mark.parametrize("prec", [ox.FP64, ox.FP32_IR])
 def test_continuous_batching_bit_identical(prec):
     """Instances join one at a time while earlier ones are mid-solve."""
     alone = [W.run_gpu(sc, precision=prec, keep_steps=True) for sc in MIX]
