@@ -1549,24 +1549,36 @@ void share_formats(MemberSet* m, const std::vector<int>& which, bool ir)
                     }
             }
             if (left == 0) break;
-            // else the format acceptable to the most remaining starters
-            std::map<int, std::pair<int, double>> votes;  // format -> (count, summed ms)
+            // else the format acceptable to the most remaining starters; ties
+            // go to the format more of them hold already (no re-layout of
+            // their values, and a system starting alone keeps its format
+            // from step to step), then to the smaller summed time
+            struct Vote {
+                int count = 0, current = 0;
+                double ms = 0.0;
+            };
+            std::map<int, Vote> votes;
             for (size_t t = 0; t < todo.size(); t++)
                 if (!done[t])
                     for (const auto& c : todo[t].ok) {
-                        auto& v = votes[c.second];
-                        v.first++;
-                        v.second += c.first;
+                        Vote& v = votes[c.second];
+                        v.count++;
+                        v.current += fmt(todo[t].s) == c.second;
+                        v.ms += c.first;
                     }
-            int pick = 0, cnt = -1;
-            double ms = 0.0;
-            for (const auto& v : votes)
-                if (v.second.first > cnt || (v.second.first == cnt && v.second.second < ms)) {
-                    pick = v.first;
-                    cnt = v.second.first;
-                    ms = v.second.second;
+            int pick = 0;
+            Vote best;
+            best.count = -1;
+            for (const auto& kv : votes) {
+                const Vote& v = kv.second;
+                if (v.count > best.count ||
+                    (v.count == best.count &&
+                     (v.current > best.current || (v.current == best.current && v.ms < best.ms)))) {
+                    pick = kv.first;
+                    best = v;
                 }
-            if (cnt <= 0) break;
+            }
+            if (best.count <= 0) break;
             inuse.push_back(pick);
         }
     }
