@@ -490,13 +490,19 @@ void build_triples(const PatternHost& h, PatternLayout& L)
     std::vector<std::vector<int>> stencils;
     std::vector<uint16_t> tid(ns);
     std::vector<int> tptr(ns + 1, 0);
-    std::vector<int> u, starts;
+    std::vector<int> u, starts, key, prev;
     int64_t total = 0;
     for (int64_t sl = 0; sl < ns; sl++) {
         u.clear();
+        prev.clear();
         const int64_t r1 = std::min<int64_t>(h.n, (sl + 1) * kSellC);
-        for (int64_t r = sl * kSellC; r < r1; r++)
-            for (int64_t k = h.ptr[r]; k < h.ptr[r + 1]; k++) u.push_back((int)(h.col[k] - r));
+        for (int64_t r = sl * kSellC; r < r1; r++) {
+            key.clear();
+            for (int64_t k = h.ptr[r]; k < h.ptr[r + 1]; k++) key.push_back((int)(h.col[k] - r));
+            if (key == prev) continue;  // neighbouring rows mostly share their offsets
+            u.insert(u.end(), key.begin(), key.end());
+            prev.swap(key);
+        }
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
         if (u.empty()) return;
@@ -528,14 +534,15 @@ void build_triples(const PatternHost& h, PatternLayout& L)
         const int64_t r1 = std::min<int64_t>(h.n, (sl + 1) * kSellC);
         for (int64_t r = sl * kSellC; r < r1; r++) {
             const int l = (int)(r - sl * kSellC);
-            const int64_t* c0 = h.col.data() + h.ptr[r];
-            const int64_t* c1 = h.col.data() + h.ptr[r + 1];
+            // the row's columns and the stencil's columns both ascend: merge
+            int64_t q = h.ptr[r];
+            const int64_t q1 = h.ptr[r + 1];
             for (size_t k = 0; k < st.size(); k++)
                 for (int t = 0; t < 3; t++) {
                     const int64_t c = r + st[k] + t;
-                    const int64_t* f = std::lower_bound(c0, c1, c);
-                    if (f != c1 && *f == c)
-                        L.tsrc[tptr[sl] + kSellC * (3 * (int64_t)k + t) + l] = (int)(f - h.col.data());
+                    while (q < q1 && h.col[q] < c) q++;
+                    if (q < q1 && h.col[q] == c)
+                        L.tsrc[tptr[sl] + kSellC * (3 * (int64_t)k + t) + l] = (int)q;
                 }
         }
     }
