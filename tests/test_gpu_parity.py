@@ -342,6 +342,71 @@ def test_full_size_quad_config_vs_oracle():
         assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < SOL_TOL
 
 
+def test_full_size_tet_heat_properties():
+    """Heat transport on 96^3 cubes x 6 tets (912,673 nodes, 13.5M nonzeros,
+    divergence-free Gauss-point velocity, grid Peclet 1-20), first
+    Crank-Nicolson step: size-independent properties. CRS nnz closed form;
+    the steady operator (conduction + advection with a divergence-free
+    velocity) annihilates constants; every SpMV format the pattern admits
+    gives the same bits; FP64 BiCGStab on the reference's row-only system and
+    the FP32-inside-FP64 refinement reach true FP64 residual <= 1e-10 and
+    agree."""
+    import scipy.sparse as sp
+    n = 96
+    sc = W.tet_heat(n, n_steps=1)
+    pat, s = W.make_system(sc)
+    q = n + 1
+    assert pat.nnz == q ** 3 + 2 * (3 * q * q * n + 3 * q * n * n + n ** 3)
+    ptr, col, _ = pat.export()
+    st = sc.steps[0]
+    s.set_vector(ox.VEC_UOLD, sc.u0)
+    s.set_vector(ox.VEC_X, sc.u0)
+    s.initialize()
+    s.assemble(st.dt, st.theta, True)
+    A0 = s.get_matrix()
+    csr = sp.csr_matrix((A0, col, ptr), shape=(sc.n, sc.n))
+    assert np.abs(csr @ np.ones(sc.n)).max() <= 1e-12 * np.abs(A0).max() * 15
+    x = np.random.default_rng(96).standard_normal(sc.n)
+    s.set_spmv_variant(5)
+    y5 = s.spmv(x)
+    tried = []
+    for f in (3, 13, 14, 17, 18, 19, 20):
+        try:
+            s.set_spmv_variant(f)
+        except ox.OgsxError:
+            continue
+        tried.append(f)
+        assert np.array_equal(s.spmv(x), y5), f
+    print(f"[full tet] formats compared with 5: {tried}", flush=True)
+    s.set_spmv_variant(0)
+    s.initialize()
+    s.assemble(st.dt, st.theta)
+    s.set_dirichlet(sc.bc_idx, sc.bc_val, ox.DIRICHLET_ROW)
+    A = s.get_matrix()
+    b = s.get_vector(ox.VEC_B)
+    csr = sp.csr_matrix((A, col, ptr), shape=(sc.n, sc.n))
+    info = s.solve(method=ox.BICGSTAB, tol=1e-10, max_iter=10000)
+    x64 = s.get_vector(ox.VEC_X)
+    r = np.linalg.norm(b - csr @ x64) / np.linalg.norm(b)
+    print(f"[full tet] iters={info['iterations']} relres={r:.3e}", flush=True)
+    assert info["converged"] == 1 and r <= 1e-10 * 1.5
+    pat2, s2 = W.make_system(sc, ox.FP32_IR)
+    s2.set_vector(ox.VEC_UOLD, sc.u0)
+    s2.set_vector(ox.VEC_X, sc.u0)
+    s2.initialize()
+    s2.assemble(st.dt, st.theta)
+    s2.set_dirichlet(sc.bc_idx, sc.bc_val, ox.DIRICHLET_ROW)
+    info2 = s2.solve(method=ox.BICGSTAB, tol=1e-10, max_iter=10000)
+    x32 = s2.get_vector(ox.VEC_X)
+    r2 = np.linalg.norm(b - csr @ x32) / np.linalg.norm(b)
+    print(f"[full tet IR] iters={info2['iterations']} outer={info2['outer']} relres={r2:.3e}",
+          flush=True)
+    assert info2["converged"] == 1 and r2 <= 1e-10
+    assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-6
+    s.close()
+    s2.close()
+
+
 def test_full_size_hex_properties():
     """128^3 hexes (2.1M nodes, 57M nonzeros): size-independent properties.
     CRS nnz closed form; symmetric elimination leaves A symmetric; the steady
