@@ -169,6 +169,11 @@ struct ogsx_system {
     // pattern on device
     int *ptr = nullptr, *col = nullptr, *diagpos = nullptr;
     int *conn = nullptr, *color_elems = nullptr, *slot = nullptr;
+    // two-pass assembly: node -> element incidence and the element buffers,
+    // allocated on the first assembly that uses them
+    int *inc_ptr = nullptr, *inc_ele = nullptr;
+    uint8_t* asm_loc = nullptr;
+    double *asm_eb = nullptr, *asm_rb = nullptr;
     // values
     double* A = nullptr;
     float* A32 = nullptr;
@@ -605,6 +610,12 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
          b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
+    // two-pass assembly: incidence, row positions, element matrices and
+    // right-hand sides
+    if (!h.inc_ptr.empty() && h.row_max <= kAsmRowMax) {
+        const int64_t np = h.npe;
+        t += b(n + 1, 4) + b(E * np, 4) + b(E * np * np, 1) + b(E * np * np, 8) + b(E * np * 2, 8);
+    }
     return t;
 }
 
@@ -796,6 +807,11 @@ int alloc_vecs(ogsx_system* s, DevVecs<T>& v)
 }  // namespace
 
 #include "solver.inc"
+
+namespace {
+int alloc_two_pass(ogsx_system* s);
+bool use_two_pass(const ogsx_system* s);
+}  // namespace
 
 // =========================================================================
 // C ABI
@@ -1120,6 +1136,7 @@ int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e)
         }
         HIP_TRY(up(s->vel, e->vel, 8 * need));
     }
+    if (use_two_pass(s)) OK_TRY(alloc_two_pass(s));
     s->elements_set = true;
     HIP_TRY(hipStreamSynchronize(s->stream));
     return OGSX_OK;
@@ -1133,11 +1150,52 @@ template <int ET, int PROC, bool ISO>
 void launch_assemble_colours(ogsx_system* s, const AsmArgs& a)
 {
     const PatternHost& h = *s->pat;
+    if (a.eb) {  // two passes: every element at once, then row by row
+        if (h.n_elem > 0)
+            k_assemble<ET, PROC, ISO, true><<<(int)((h.n_elem + 63) / 64), 64, 0, s->stream>>>(
+                a, 0, (int)h.n_elem);
+        const int grav = PROC == OGSX_LIQUID_FLOW && a.rho_g != 0.0;
+        k_assemble_rows<ET><<<(s->n + kAsmRowBlock - 1) / kAsmRowBlock, kAsmRowBlock, 0, s->stream>>>(
+            a, s->inc_ptr, s->inc_ele, s->asm_loc, s->ptr, s->n, grav);
+        return;
+    }
     for (int c = 0; c < h.n_colors; c++) {
         const int off = h.color_ptr[c], cnt = h.color_ptr[c + 1] - off;
         if (cnt == 0) continue;
-        k_assemble<ET, PROC, ISO><<<(cnt + 63) / 64, 64, 0, s->stream>>>(a, off, cnt);
+        k_assemble<ET, PROC, ISO, false><<<(cnt + 63) / 64, 64, 0, s->stream>>>(a, off, cnt);
     }
+}
+
+// The two-pass assembly's arrays (incidence lists; element matrices and
+// right-hand sides), allocated and uploaded once.
+int alloc_two_pass(ogsx_system* s)
+{
+    if (s->asm_eb) return OGSX_OK;
+    const PatternHost& h = *s->pat;
+    const int64_t npe = h.npe;
+    const int64_t cnt = h.n_elem * npe * npe;
+    OK_TRY(dalloc(s, &s->inc_ptr, h.inc_ptr.size()));
+    OK_TRY(dalloc(s, &s->inc_ele, h.inc_ele.size()));
+    OK_TRY(dalloc(s, &s->asm_loc, (size_t)cnt));
+    OK_TRY(dalloc(s, &s->asm_rb, (size_t)(h.n_elem * npe * 2)));
+    OK_TRY(dalloc(s, &s->asm_eb, (size_t)cnt));
+    HIP_TRY(hipMemcpyAsync(s->inc_ptr, h.inc_ptr.data(), 4 * h.inc_ptr.size(), hipMemcpyHostToDevice,
+                           s->stream));
+    HIP_TRY(hipMemcpyAsync(s->inc_ele, h.inc_ele.data(), 4 * h.inc_ele.size(), hipMemcpyHostToDevice,
+                           s->stream));
+    if (cnt > 0)
+        k_row_positions<<<blocks_for(cnt, 256 * 4), 256, 0, s->stream>>>(cnt, (int)npe, s->conn, s->slot,
+                                                                          s->ptr, s->asm_loc);
+    HIP_TRY(hipGetLastError());
+    return OGSX_OK;
+}
+
+// Two-pass assembly where every CRS row fits the row pass's LDS accumulator;
+// the coloured scatter otherwise (and for patterns with 2^31 or more
+// element-node pairs, which have no incidence list).
+bool use_two_pass(const ogsx_system* s)
+{
+    return !s->pat->inc_ptr.empty() && s->pat->row_max <= kAsmRowMax;
 }
 
 template <int ET>
@@ -1280,6 +1338,10 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
     a.rho_g = s->rho_g;
     a.dt = dt;
     a.theta = theta;
+    if (s->asm_eb) {  // two-pass assembly (buffers from ogsx_system_set_elements)
+        a.eb = s->asm_eb;
+        a.rb = s->asm_rb;
+    }
     int pk = prof_begin(s, OGSX_PROF_ASSEMBLY);
     switch (s->pat->etype) {
         case OGSX_LINE2: launch_assemble_et<OGSX_LINE2>(s, a); break;

@@ -663,6 +663,8 @@ struct AsmArgs {
     const double* u_old;
     double* A;
     double* b;
+    double* eb;   // element matrices [E][npe][npe] (two-pass assembly)
+    double* rb;   // element right-hand sides [E][npe][2]: (rhs, gravity)
     int cond_stride;
     int ngp;
     int steady;
@@ -760,13 +762,13 @@ __device__ __forceinline__ double elem_gradients(const double* X, const double* 
 // an isotropic (diagonal) tensor: the zero off-diagonal products of the
 // reference's k,l loop add exact zeros and are skipped, and K is formed on
 // its lower triangle and mirrored.
-template <int ET, int PROC, bool ISO>
+template <int ET, int PROC, bool ISO, bool STORE>
 __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
 {
     constexpr int NPE = ETraits<ET>::NPE, D = ETraits<ET>::D, NW = ETraits<ET>::NW;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    const int e = a.color_elems[off + t];
+    const int e = STORE ? off + t : a.color_elems[off + t];
     int nd[NPE];
     double X[NPE], Y[NPE], Z[NPE];
 #pragma unroll
@@ -908,6 +910,20 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
             rhs[i] = val;
         }
     }
+    if (STORE) {
+        // two-pass assembly: the element's matrix and right-hand side, for
+        // k_assemble_rows to add row by row
+        double* eb = a.eb + (int64_t)e * NPE * NPE;
+        double* rb = a.rb + (int64_t)e * NPE * 2;
+#pragma unroll
+        for (int k = 0; k < NPE * NPE; k++) eb[k] = lhs[k];
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+            rb[2 * i] = rhs[i];
+            rb[2 * i + 1] = do_grav ? G[i] : 0.0;
+        }
+        return;
+    }
     // atomic-free scatter: elements of one colour own disjoint rows
     const int32_t* sl = a.slot + (int64_t)e * NPE * NPE;
 #pragma unroll
@@ -916,6 +932,67 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
         if (do_grav) a.b[nd[i]] += G[i];
 #pragma unroll
         for (int j = 0; j < NPE; j++) a.A[sl[i * NPE + j]] += lhs[i * NPE + j];
+    }
+}
+
+// Second pass of the two-pass assembly: one thread per row r adds the rows
+// of the element matrices that belong to node r, element by element in
+// ascending element order (the order of the reference's element loop,
+// FEM/rf_pcs.cpp:5741-5762), and within an element by local node; no two
+// threads touch one entry, so no colouring and no atomics.  The row is
+// accumulated in LDS (loc: an entry's position in its row) starting from
+// A's current values, so every entry is summed in the same order as
+// A += contribution element by element.  b gets the element right-hand
+// side, then the gravity term, as the scatter does.
+constexpr int kAsmRowMax = 32;     // longest row the two-pass assembly takes
+constexpr int kAsmRowBlock = 128;  // threads (rows) per block
+template <int ET>
+__global__ __launch_bounds__(kAsmRowBlock) void k_assemble_rows(AsmArgs a, const int32_t* inc_ptr,
+                                                                const int32_t* inc_ei,
+                                                                const uint8_t* loc, const int32_t* ptr,
+                                                                int n, int grav)
+{
+    constexpr int NPE = ETraits<ET>::NPE;
+    __shared__ double acc[kAsmRowMax * kAsmRowBlock];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * kAsmRowBlock + tid;
+    if (r >= n) return;
+    const OGSX_GL double* eb = gl(a.eb);
+    const OGSX_GL double* rb = gl(a.rb);
+    const OGSX_GL uint8_t* lc = gl(loc);
+    OGSX_GL double* A = gl(a.A);
+    const int p0 = ptr[r], len = ptr[r + 1] - p0;
+    for (int c = 0; c < len; c++) acc[c * kAsmRowBlock + tid] = A[p0 + c];
+    double br = gl(a.b)[r];
+    const int q1 = inc_ptr[r + 1];
+    for (int q = inc_ptr[r]; q < q1; q++) {
+        const int64_t ei = inc_ei[q];
+        const int64_t base = ei * NPE;
+        double v[NPE];
+        int l[NPE];
+#pragma unroll
+        for (int j = 0; j < NPE; j++) {
+            v[j] = eb[base + j];
+            l[j] = lc[base + j];
+        }
+#pragma unroll
+        for (int j = 0; j < NPE; j++) acc[l[j] * kAsmRowBlock + tid] += v[j];
+        br += rb[2 * ei];
+        if (grav) br += rb[2 * ei + 1];
+    }
+    for (int c = 0; c < len; c++) A[p0 + c] = acc[c * kAsmRowBlock + tid];
+    gl(a.b)[r] = br;
+}
+
+// Position of every element-matrix entry in its CRS row (two-pass assembly).
+__global__ __launch_bounds__(256) void k_row_positions(int64_t count, int npe, const int32_t* conn,
+                                                        const int32_t* slot, const int32_t* ptr,
+                                                        uint8_t* loc)
+{
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * 256) {
+        const int r = conn[t / npe];  // t = (e npe + i) npe + j
+        loc[t] = (uint8_t)(slot[t] - ptr[r]);
     }
 }
 

@@ -58,6 +58,11 @@ struct PatternHost {
     std::vector<int32_t> color_ptr;        // [n_colors+1]
     std::vector<int32_t> color_elems;      // [n_elem] element ids by colour
     int n_colors = 0;
+    // node -> (element, local node) incidence e * npe + i, ascending (an
+    // element with a repeated node is listed once per occurrence)
+    std::vector<int32_t> inc_ptr;          // [n+1]
+    std::vector<int32_t> inc_ele;
+    int64_t row_max = 0;                   // longest CRS row
 };
 
 // Builds everything in PatternHost; returns OGSX_OK or an error with msg.

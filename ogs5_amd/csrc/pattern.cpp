@@ -94,6 +94,14 @@ int build_pattern(int64_t n_nodes, int64_t n_elem, int etype,
                 if (!dup) iele[fill[P.conn[e * npe + a]]++] = (int32_t)e;
             }
     }
+    if (n_elem * npe < (int64_t(1) << 31)) {
+        P.inc_ptr.assign(n_nodes + 1, 0);
+        for (int64_t k = 0; k < n_elem * npe; k++) P.inc_ptr[P.conn[k] + 1]++;
+        for (int64_t i = 0; i < n_nodes; i++) P.inc_ptr[i + 1] += P.inc_ptr[i];
+        P.inc_ele.resize(n_elem * npe);
+        std::vector<int32_t> fill(P.inc_ptr.begin(), P.inc_ptr.end() - 1);
+        for (int64_t k = 0; k < n_elem * npe; k++) P.inc_ele[fill[P.conn[k]]++] = (int32_t)k;
+    }
     // row lengths, then columns
     std::vector<int64_t> rowlen(n_nodes);
     auto gather = [&](int64_t i, std::vector<int32_t>& buf) {
@@ -113,7 +121,10 @@ int build_pattern(int64_t n_nodes, int64_t n_elem, int etype,
         }
     });
     P.ptr.assign(n_nodes + 1, 0);
-    for (int64_t i = 0; i < n_nodes; i++) P.ptr[i + 1] = P.ptr[i] + rowlen[i];
+    for (int64_t i = 0; i < n_nodes; i++) {
+        P.ptr[i + 1] = P.ptr[i] + rowlen[i];
+        P.row_max = std::max(P.row_max, rowlen[i]);
+    }
     P.nnz = P.ptr[n_nodes];
     if (P.nnz >= (int64_t(1) << 31)) {
         msg = "nnz exceeds the 32-bit index range";
