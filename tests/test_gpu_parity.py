@@ -112,6 +112,40 @@ def test_assembly_parity(name, bc):
     assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
 
 
+def _star_mesh(m):
+    """Line elements from a hub node 0 to each of nodes 1..m, plus the chain
+    1-2-...-m: the hub's CRS row holds m + 1 entries."""
+    from ogs5_amd import mesh as M
+    k = np.arange(1, m + 1)
+    x = np.r_[0.0, 0.1 * k + 0.013 * np.sin(k)]
+    conn = np.array([(0, i) for i in k] + [(i, i + 1) for i in range(1, m)], dtype=np.int64)
+    z = np.zeros_like(x)
+    return M.Mesh(M.LINE2, 1, x, z.copy(), z.copy(), conn, (len(conn),), (0.1,),
+                  np.arange(len(conn), dtype=np.int64))
+
+
+@pytest.mark.parametrize("process", [ox.LIQUID_FLOW, ox.HEAT_TRANSPORT])
+def test_assembly_parity_long_rows(process):
+    """Rows longer than the two-pass assembly's 32-entry row accumulator
+    (a hub node in 40 line elements) take the coloured scatter: A and b
+    against the oracle, with gravity (flow) and advection (heat)."""
+    m = _star_mesh(40)
+    rng = np.random.default_rng(9)
+    E = m.n_elem
+    flow = process == ox.LIQUID_FLOW
+    vel = None if flow else rng.uniform(-1e-6, 1e-6, (E, 3, 3))
+    sc = W.Scenario("star", m, process, np.full(E, 1e-4), 1e-5 * np.exp(rng.normal(0.0, 1.0, E)), 1,
+                    np.array([m.n_nodes - 1], dtype=np.int64), np.array([1.0]),
+                    np.zeros(0, np.int64), np.zeros(0), rng.standard_normal(m.n_nodes),
+                    [W.Step(10.0, 1.0 if flow else 0.5)], mu=1e-3 if flow else 1.0,
+                    rho_g=9810.0 if flow else 0.0, adv_coef=0.0 if flow else 4.2e6, vel=vel)
+    A, b, Ar, br, ptr, s, eq = _assemble_both(sc, None)
+    assert np.diff(ptr).max() > 32
+    assert _row_scaled_close(A, Ar, ptr, ENTRY_TOL) < ENTRY_TOL
+    assert np.abs(b - br).max() <= ENTRY_TOL * max(np.abs(br).max(), 1e-300)
+    s.close()
+
+
 @pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 17, 18, 19, 20,
                                  4, 8, 16, 32, 64])
 def test_spmv_parity_all_variants(lpr):
