@@ -171,7 +171,7 @@ struct ogsx_system {
     int *conn = nullptr, *color_elems = nullptr, *slot = nullptr;
     // two-pass assembly: node -> element incidence and the element buffers,
     // allocated on the first assembly that uses them
-    int *inc_ptr = nullptr, *inc_ele = nullptr;
+    int *inc_ptr = nullptr, *inc_ele = nullptr, *asm_pos = nullptr;
     uint8_t* asm_loc = nullptr;
     double *asm_eb = nullptr, *asm_rb = nullptr;
     // values
@@ -614,7 +614,7 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
     // right-hand sides
     if (!h.inc_ptr.empty() && h.row_max <= kAsmRowMax) {
         const int64_t np = h.npe;
-        t += b(n + 1, 4) + b(E * np, 4) + b(E * np * np, 1) + b(E * np * np, 8) + b(E * np * 2, 8);
+        t += b(n + 1, 4) + 2 * b(E * np, 4) + b(E * np * np, 1) + b(E * np * np, 8) + b(E * np * 2, 8);
     }
     return t;
 }
@@ -1156,7 +1156,7 @@ void launch_assemble_colours(ogsx_system* s, const AsmArgs& a)
                 a, 0, (int)h.n_elem);
         const int grav = PROC == OGSX_LIQUID_FLOW && a.rho_g != 0.0;
         k_assemble_rows<ET><<<(s->n + kAsmRowBlock - 1) / kAsmRowBlock, kAsmRowBlock, 0, s->stream>>>(
-            a, s->inc_ptr, s->inc_ele, s->asm_loc, s->ptr, s->n, grav);
+            a, s->inc_ptr, s->asm_loc, s->ptr, s->n, grav);
         return;
     }
     for (int c = 0; c < h.n_colors; c++) {
@@ -1176,6 +1176,7 @@ int alloc_two_pass(ogsx_system* s)
     const int64_t cnt = h.n_elem * npe * npe;
     OK_TRY(dalloc(s, &s->inc_ptr, h.inc_ptr.size()));
     OK_TRY(dalloc(s, &s->inc_ele, h.inc_ele.size()));
+    OK_TRY(dalloc(s, &s->asm_pos, h.inc_ele.size()));
     OK_TRY(dalloc(s, &s->asm_loc, (size_t)cnt));
     OK_TRY(dalloc(s, &s->asm_rb, (size_t)(h.n_elem * npe * 2)));
     OK_TRY(dalloc(s, &s->asm_eb, (size_t)cnt));
@@ -1183,9 +1184,10 @@ int alloc_two_pass(ogsx_system* s)
                            s->stream));
     HIP_TRY(hipMemcpyAsync(s->inc_ele, h.inc_ele.data(), 4 * h.inc_ele.size(), hipMemcpyHostToDevice,
                            s->stream));
-    if (cnt > 0)
-        k_row_positions<<<blocks_for(cnt, 256 * 4), 256, 0, s->stream>>>(cnt, (int)npe, s->conn, s->slot,
-                                                                          s->ptr, s->asm_loc);
+    const int64_t ni = (int64_t)h.inc_ele.size();
+    if (ni > 0)
+        k_row_positions<<<blocks_for(ni, 256 * 4), 256, 0, s->stream>>>(
+            ni, (int)npe, s->inc_ele, s->conn, s->slot, s->ptr, s->asm_pos, s->asm_loc);
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
 }
@@ -1341,6 +1343,7 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
     if (s->asm_eb) {  // two-pass assembly (buffers from ogsx_system_set_elements)
         a.eb = s->asm_eb;
         a.rb = s->asm_rb;
+        a.pos = s->asm_pos;
     }
     int pk = prof_begin(s, OGSX_PROF_ASSEMBLY);
     switch (s->pat->etype) {

@@ -663,8 +663,9 @@ struct AsmArgs {
     const double* u_old;
     double* A;
     double* b;
-    double* eb;   // element matrices [E][npe][npe] (two-pass assembly)
-    double* rb;   // element right-hand sides [E][npe][2]: (rhs, gravity)
+    double* eb;         // two-pass assembly: element rows [E*npe][npe] and
+    double* rb;         // right-hand sides [E*npe][2] (rhs, gravity), stored
+    const int32_t* pos; // at the (element, node) pair's incidence position pos
     int cond_stride;
     int ngp;
     int steady;
@@ -911,16 +912,16 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
         }
     }
     if (STORE) {
-        // two-pass assembly: the element's matrix and right-hand side, for
-        // k_assemble_rows to add row by row
-        double* eb = a.eb + (int64_t)e * NPE * NPE;
-        double* rb = a.rb + (int64_t)e * NPE * 2;
-#pragma unroll
-        for (int k = 0; k < NPE * NPE; k++) eb[k] = lhs[k];
+        // two-pass assembly: each element row goes to its node's incidence
+        // list (node-major), where k_assemble_rows reads it in sequence
 #pragma unroll
         for (int i = 0; i < NPE; i++) {
-            rb[2 * i] = rhs[i];
-            rb[2 * i + 1] = do_grav ? G[i] : 0.0;
+            const int64_t q = a.pos[(int64_t)e * NPE + i];
+            double* eb = a.eb + q * NPE;
+#pragma unroll
+            for (int j = 0; j < NPE; j++) eb[j] = lhs[i * NPE + j];
+            a.rb[2 * q] = rhs[i];
+            a.rb[2 * q + 1] = do_grav ? G[i] : 0.0;
         }
         return;
     }
@@ -939,7 +940,9 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
 // of the element matrices that belong to node r, element by element in
 // ascending element order (the order of the reference's element loop,
 // FEM/rf_pcs.cpp:5741-5762), and within an element by local node; no two
-// threads touch one entry, so no colouring and no atomics.  The row is
+// threads touch one entry, so no colouring and no atomics.  The element
+// rows sit in node r's incidence order, so a thread reads one contiguous
+// stretch and neighbouring rows read neighbouring stretches.  The row is
 // accumulated in LDS (loc: an entry's position in its row) starting from
 // A's current values, so every entry is summed in the same order as
 // A += contribution element by element.  b gets the element right-hand
@@ -948,7 +951,6 @@ constexpr int kAsmRowMax = 32;     // longest row the two-pass assembly takes
 constexpr int kAsmRowBlock = 128;  // threads (rows) per block
 template <int ET>
 __global__ __launch_bounds__(kAsmRowBlock) void k_assemble_rows(AsmArgs a, const int32_t* inc_ptr,
-                                                                const int32_t* inc_ei,
                                                                 const uint8_t* loc, const int32_t* ptr,
                                                                 int n, int grav)
 {
@@ -965,9 +967,8 @@ __global__ __launch_bounds__(kAsmRowBlock) void k_assemble_rows(AsmArgs a, const
     for (int c = 0; c < len; c++) acc[c * kAsmRowBlock + tid] = A[p0 + c];
     double br = gl(a.b)[r];
     const int q1 = inc_ptr[r + 1];
-    for (int q = inc_ptr[r]; q < q1; q++) {
-        const int64_t ei = inc_ei[q];
-        const int64_t base = ei * NPE;
+    for (int64_t q = inc_ptr[r]; q < q1; q++) {
+        const int64_t base = q * NPE;
         double v[NPE];
         int l[NPE];
 #pragma unroll
@@ -977,22 +978,27 @@ __global__ __launch_bounds__(kAsmRowBlock) void k_assemble_rows(AsmArgs a, const
         }
 #pragma unroll
         for (int j = 0; j < NPE; j++) acc[l[j] * kAsmRowBlock + tid] += v[j];
-        br += rb[2 * ei];
-        if (grav) br += rb[2 * ei + 1];
+        br += rb[2 * q];
+        if (grav) br += rb[2 * q + 1];
     }
     for (int c = 0; c < len; c++) A[p0 + c] = acc[c * kAsmRowBlock + tid];
     gl(a.b)[r] = br;
 }
 
-// Position of every element-matrix entry in its CRS row (two-pass assembly).
-__global__ __launch_bounds__(256) void k_row_positions(int64_t count, int npe, const int32_t* conn,
-                                                        const int32_t* slot, const int32_t* ptr,
+// Two-pass assembly tables, per incidence position q (node-major, inc_ei[q] =
+// e npe + i): pos[e npe + i] = q, and loc[q npe + j] = the position of entry
+// (i, j) of element e in its CRS row.
+__global__ __launch_bounds__(256) void k_row_positions(int64_t count, int npe, const int32_t* inc_ei,
+                                                        const int32_t* conn, const int32_t* slot,
+                                                        const int32_t* ptr, int32_t* pos,
                                                         uint8_t* loc)
 {
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < count;
-         t += (int64_t)gridDim.x * 256) {
-        const int r = conn[t / npe];  // t = (e npe + i) npe + j
-        loc[t] = (uint8_t)(slot[t] - ptr[r]);
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < count;
+         q += (int64_t)gridDim.x * 256) {
+        const int64_t ei = inc_ei[q];
+        const int r = conn[ei];
+        pos[ei] = (int32_t)q;
+        for (int j = 0; j < npe; j++) loc[q * npe + j] = (uint8_t)(slot[ei * npe + j] - ptr[r]);
     }
 }
 
