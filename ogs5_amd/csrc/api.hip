@@ -428,6 +428,14 @@ int sync_sell(ogsx_system* s, bool want64, bool want32)
 {
     const Grp g1{s->dev, nullptr, nullptr};
     const int lay64 = is_tri(s->fmt64) ? 1 : 0, lay32 = is_tri(s->fmt32) ? 1 : 0;
+    const bool stale64 = want64 && (s->sval64_version != s->a_version || s->sval64_layout != lay64);
+    const bool stale32 =
+        want32 && s->sval32 && (s->sval32_version != s->a_version || s->sval32_layout != lay32);
+    if (stale64 && stale32 && lay64 == lay32) {
+        k_to_sell_both_g<<<s->G, kBlock, 0, s->stream>>>(g1, lay64);
+        s->sval64_version = s->sval32_version = s->a_version;
+        s->sval64_layout = s->sval32_layout = lay64;
+    }
     if (want64 && (s->sval64_version != s->a_version || s->sval64_layout != lay64)) {
         k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1, lay64);
         s->sval64_version = s->a_version;

@@ -1306,6 +1306,24 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g, int tri)
     });
 }
 
+// Both value copies of an IR system in one pass when they share a layout:
+// A read once, the FP64 and the FP32 value written.
+__global__ __launch_bounds__(kBlock) void k_to_sell_both_g(Grp g, int tri)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    OGSX_GL double* sv64 = gl(S.sval64);
+    OGSX_GL float* sv32 = gl(S.sval32);
+    const OGSX_GL int* src = gl(tri ? S.tri_src : S.sell_src);
+    const OGSX_GL double* A = gl(S.A);
+    vec_vblocks(c, tri ? S.npad3 : S.npad, [&](int p) {
+        const int k = src[p];
+        const double v = k >= 0 ? A[k] : 0.0;
+        sv64[p] = v;
+        sv32[p] = (float)v;
+    });
+}
+
 __global__ __launch_bounds__(kBlock) void k_cast_f32_g(Grp g)
 {
     const Ctx c = ctx_of(g);
