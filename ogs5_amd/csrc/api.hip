@@ -1527,8 +1527,8 @@ int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx, const doub
     } else {
         k_bc_mark<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d,
                                                                    s->bcflag, s->bcg, (int8_t)1);
-        k_bc_sym<<<blocks_for(s->n, kBlock), kBlock, 0, s->stream>>>(s->n, s->ptr, s->col, s->diagpos,
-                                                                     s->bcflag, s->bcg, s->A, s->v64.b);
+        k_bc_sym<<<blocks_for(s->n, kBlock / 32), kBlock, 0, s->stream>>>(
+            s->n, s->ptr, s->col, s->diagpos, s->bcflag, s->bcg, s->A, s->v64.b);
         k_bc_mark<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d,
                                                                    s->bcflag, s->bcg, (int8_t)0);
     }

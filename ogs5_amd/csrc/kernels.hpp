@@ -2090,29 +2090,45 @@ __global__ __launch_bounds__(kBlock) void k_bc_row(int m, const int* idx, const 
 
 // Symmetric elimination: Dirichlet rows as above; every other row lifts
 // b_i -= a_ik g_k and zeroes a_ik for Dirichlet columns k (column order).
+// 32 lanes per row: the row's columns are read in one coalesced pass, and
+// the lifts are applied one Dirichlet column at a time in column order
+// (ballot over the lanes that hold one, then each lane's product in turn).
 __global__ __launch_bounds__(kBlock) void k_bc_sym(int n, const int* ptr, const int* col,
                                                    const int* diagpos, const int8_t* flag,
                                                    const double* g, double* A, double* b)
 {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int lane = threadIdx.x & 31;
+    const int half = (threadIdx.x & 63) >> 5;
+    const int step = gridDim.x * (kBlock / 32);
+    for (int i = blockIdx.x * (kBlock / 32) + (int)(threadIdx.x >> 5); i < n; i += step) {
+        const int p0 = ptr[i], p1 = ptr[i + 1];
         if (flag[i]) {
             const int dp = diagpos[i];
-            const double vdiag = dp >= 0 ? A[dp] : 0.0;
-            for (int j = ptr[i]; j < ptr[i + 1]; j++)
+            const double vdiag = dp >= 0 ? A[dp] : 0.0;  // the diagonal is not zeroed
+            for (int j = p0 + lane; j < p1; j += 32)
                 if (col[j] != i) A[j] = 0.0;
-            b[i] = vdiag * g[i];
+            if (lane == 0) b[i] = vdiag * g[i];
         } else {
             double bi = b[i];
             bool touched = false;
-            for (int j = ptr[i]; j < ptr[i + 1]; j++) {
-                const int c = col[j];
-                if (flag[c]) {
-                    bi -= A[j] * g[c];
-                    A[j] = 0.0;
-                    touched = true;
+            for (int base = p0; base < p1; base += 32) {
+                const int j = base + lane;
+                const int c = j < p1 ? col[j] : 0;
+                const bool f = j < p1 && flag[c];
+                const double av = f ? A[j] : 0.0;
+                const double gv = f ? g[c] : 0.0;
+                unsigned m = (unsigned)(__ballot(f) >> (32 * half));
+                touched |= m != 0;
+                while (m) {
+                    const int l = __ffs(m) - 1;
+                    m &= m - 1;
+                    const double al = __shfl(av, l, 32);
+                    const double gl_ = __shfl(gv, l, 32);
+                    bi -= al * gl_;
                 }
+                if (f) A[j] = 0.0;
             }
-            if (touched) b[i] = bi;
+            if (lane == 0 && touched) b[i] = bi;
         }
     }
 }
