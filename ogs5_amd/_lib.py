@@ -20,7 +20,7 @@ BICGSTAB, CG = 2, 5
 PRECOND_NONE, PRECOND_JACOBI = 0, 1
 FP64, FP32_IR = 0, 1
 DIRICHLET_ROW, DIRICHLET_SYMMETRIC = 0, 1
-VEC_X, VEC_B, VEC_UOLD = 0, 1, 2
+VEC_X, VEC_B, VEC_UOLD, VEC_IC = 0, 1, 2, 3
 PROF_SPMV64, PROF_SPMV32, PROF_ASSEMBLY, PROF_VELOCITY = 0, 1, 2, 3
 
 

@@ -178,6 +178,7 @@ struct ogsx_system {
     double* A = nullptr;
     float* A32 = nullptr;
     double* uold = nullptr;
+    double* ic = nullptr;  // OGSX_VEC_IC: a stored node-value vector
     DevVecs<double> v64;
     DevVecs<float> v32;
     double* r64 = nullptr;  // IR residual
@@ -599,7 +600,7 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
     const int ed = edim_of(h.etype);
     int64_t t = b(n + 1, 4) + b(nnz, 4) + b(n, 4) + b((int64_t)h.conn.size(), 4) +
                 b((int64_t)h.color_elems.size(), 4) + b((int64_t)h.conn.size() * h.npe, 4) + b(nnz, 8) +
-                b(n, 8) + 12 * b(n, 8) + b(16 * G, 8) + b(8, 8) + b(1, sizeof(KrylovState)) +
+                2 * b(n, 8) + 12 * b(n, 8) + b(16 * G, 8) + b(8, 8) + b(1, sizeof(KrylovState)) +
                 b(n, 1) + b(n, 8) + b(1, sizeof(SysDev)) + b(1, sizeof(KrylovParams));
     if (precision == OGSX_FP32_IR) t += b(nnz, 4) + 12 * b(n, 4) + b(n, 8) + b(1, sizeof(IRState));
     if (!L.sell16)  // tiles and 32-bit SELL columns, built at creation only then
@@ -960,7 +961,8 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
         (rc = dalloc(s, &s->diagpos, h.n)) || (rc = dalloc(s, &s->conn, h.conn.size())) ||
         (rc = dalloc(s, &s->color_elems, h.color_elems.size())) ||
         (rc = dalloc(s, &s->slot, h.conn.size() * h.npe)) || (rc = dalloc(s, &s->A, h.nnz)) ||
-        (rc = dalloc(s, &s->uold, h.n)) || (rc = alloc_vecs(s, s->v64)) ||
+        (rc = dalloc(s, &s->uold, h.n)) || (rc = dalloc(s, &s->ic, h.n)) ||
+        (rc = alloc_vecs(s, s->v64)) ||
         (rc = dalloc(s, &s->part, (size_t)16 * s->G)) || (rc = dalloc(s, &s->red, 8)) ||
         (rc = dalloc(s, &s->st, 1)) || (rc = dalloc(s, &s->bcflag, h.n)) ||
         (rc = dalloc(s, &s->bcg, h.n)))
@@ -1296,6 +1298,7 @@ int check_vec(ogsx_system* s, int which, double** p)
         case OGSX_VEC_X: *p = s->v64.x; return OGSX_OK;
         case OGSX_VEC_B: *p = s->v64.b; return OGSX_OK;
         case OGSX_VEC_UOLD: *p = s->uold; return OGSX_OK;
+        case OGSX_VEC_IC: *p = s->ic; return OGSX_OK;
     }
     return fail(OGSX_EINVAL, "unknown vector id");
 }

@@ -287,25 +287,35 @@ def make_system(sc: Scenario, precision: int = L.FP64):
 
 def run_gpu(sc: Scenario, *, method: int = L.CG, dirichlet: int = L.DIRICHLET_SYMMETRIC,
             precision: int = L.FP64, inner_tol: Optional[float] = None, system=None,
-            keep_steps=False, tol: Optional[float] = None, ir_mode: Optional[int] = None) -> dict:
+            keep_steps=False, tol: Optional[float] = None, ir_mode: Optional[int] = None,
+            ic_on_device: bool = False, fetch_x: bool = True) -> dict:
     """Runs every step of the scenario on the GPU; returns the final x, the
-    per-step solve info and (keep_steps) every step's solution."""
+    per-step solve info and (keep_steps) every step's solution.
+
+    ic_on_device: the initial condition is already in the system's stored
+    vector (VEC_IC, set once by the caller) and is copied from there on the
+    device, so the run makes no host-to-device transfer of node values.
+    fetch_x=False leaves the final solution on the device (x is None)."""
     if system is None:
         pat, s = make_system(sc, precision)
     else:
         pat, s = system
-    s.set_vector(L.VEC_UOLD, sc.u0)
-    s.set_vector(L.VEC_X, sc.u0)
+    if ic_on_device:
+        s.copy_vector(L.VEC_UOLD, L.VEC_IC)
+        s.copy_vector(L.VEC_X, L.VEC_IC)
+    else:
+        s.set_vector(L.VEC_UOLD, sc.u0)
+        s.set_vector(L.VEC_X, sc.u0)
     infos, xs = [], []
     for k, st in enumerate(sc.steps):
-        _prepare_step(sc, s, k, dirichlet)
+        _prepare_step(sc, s, k, dirichlet, ic_on_device)
         info = s.solve(method=method, tol=sc.tol if tol is None else tol,
                        max_iter=sc.max_iter, inner_tol=inner_tol, ir_mode=ir_mode)
         infos.append(info)
         s.copy_vector(L.VEC_UOLD, L.VEC_X)
         if keep_steps:
             xs.append(s.get_vector(L.VEC_X))
-    x = s.get_vector(L.VEC_X)
+    x = s.get_vector(L.VEC_X) if fetch_x else None
     out = dict(x=x, info=infos, system=(pat, s))
     if keep_steps:
         out["steps"] = xs
@@ -356,11 +366,15 @@ def make_coupled_systems(flow: Scenario, heat: Scenario, precision: int = L.FP64
     return pat, sf, sh
 
 
-def _prepare_step(sc: Scenario, s, k: int, dirichlet: int):
+def _prepare_step(sc: Scenario, s, k: int, dirichlet: int, ic_on_device: bool = False):
     st = sc.steps[k]
     if st.reset_to_ic:
-        s.set_vector(L.VEC_UOLD, sc.u0)
-        s.set_vector(L.VEC_X, sc.u0)
+        if ic_on_device:
+            s.copy_vector(L.VEC_UOLD, L.VEC_IC)
+            s.copy_vector(L.VEC_X, L.VEC_IC)
+        else:
+            s.set_vector(L.VEC_UOLD, sc.u0)
+            s.set_vector(L.VEC_X, sc.u0)
     s.initialize()
     s.assemble(st.dt, st.theta, st.steady)
     if sc.st_idx.size:

@@ -221,6 +221,24 @@ def test_solve_with_slice_triples(prec, fmt):
     s.close()
 
 
+def test_stored_initial_condition_on_device():
+    """A job restarted from the stored vector (OGSX_VEC_IC, copied on the
+    device, including the transient restart after the steady step) gives
+    bit-for-bit the job that uploads u0 from the host, twice in a row."""
+    sc = SCEN["hex"]
+    assert any(st.reset_to_ic for st in sc.steps)
+    pat, s = W.make_system(sc, ox.FP64)
+    kw = dict(method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=ox.FP64, system=(pat, s))
+    host = W.run_gpu(sc, **kw)["x"]
+    s.set_vector(ox.VEC_IC, sc.u0)
+    assert np.array_equal(s.get_vector(ox.VEC_IC), sc.u0)
+    for _ in range(2):
+        out = W.run_gpu(sc, ic_on_device=True, fetch_x=False, **kw)
+        assert out["x"] is None
+        assert np.array_equal(s.get_vector(ox.VEC_X), host)
+    s.close()
+
+
 def test_stencil_dictionary_needs_repeated_patterns():
     """A pattern with more than 255 distinct column-offset lists has no
     dictionary: the formats are refused, the others still work."""
