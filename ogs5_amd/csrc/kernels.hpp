@@ -81,17 +81,36 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], OGSX_GL double* 
 
 // Sums G partials per value in a fixed order; every thread gets the result.
 // G is the grid size of the producer, which equals this kernel's grid.
+// Thread t adds partials t, t + kBlock, t + 2 kBlock, ... in that order.  The
+// first kPartialBatch of them per value are all loaded before the first add,
+// so a block waits for one round trip to L2 instead of one per partial (the
+// partials were written by blocks on every XCD and miss the local L2).
+constexpr int kPartialBatch = 2048 / kBlock;
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, int G,
                                                 double (&out)[NV])
 {
     __shared__ double sm[NV][kBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double t[NV][kPartialBatch];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+        for (int j = 0; j < kPartialBatch; j++) {
+            // clamped index: every load is issued unpredicated (G >= 1)
+            const int i = threadIdx.x + j * kBlock;
+            const double x = partials[(size_t)k * G + (i < G ? i : G - 1)];
+            t[k][j] = i < G ? x : 0.0;
+        }
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
+        // s starts at +0.0, so adding +0.0 for a missing partial changes no bit
         double s = 0.0;
-        for (int i = threadIdx.x; i < G; i += kBlock) s += partials[(size_t)k * G + i];
+#pragma unroll
+        for (int j = 0; j < kPartialBatch; j++) s += t[k][j];
+        for (int i = threadIdx.x + kPartialBatch * kBlock; i < G; i += kBlock)
+            s += partials[(size_t)k * G + i];
         v[k] = wave_sum(s);
     }
     if (lane == 0)
