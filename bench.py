@@ -6,8 +6,10 @@ One bench step is the whole job of that configuration: a steady solve plus
 5 implicit-Euler transient steps, each = Initialize + element assembly +
 Dirichlet + Jacobi-PCG to ||r||/||b|| <= 1e-10 (true FP64 residual).  It is
 run twice per step: plain FP64 and FP32 matrix/SpMV/preconditioner inside
-FP64 iterative refinement.  Inputs (mesh, K field, pattern) are uploaded to
-HBM before the timed region.
+FP64 iterative refinement.  Inputs (mesh, K field, pattern, initial
+condition) are uploaded to HBM before the timed region; the jobs copy the
+initial condition on the device and leave the solution there.  The same jobs
+with u0 and x crossing PCIe are timed after them (`pcie_inclusive`).
 
 Prints one JSON line.  value = MDOF-solves/s of the faster valid solver
 (both reach 1e-10 FP64 residual); the plain FP64 figures ride along.
