@@ -118,6 +118,9 @@ int ogsx_pattern_colors(const ogsx_pattern* p, int32_t* color);
  * caller that admits work by HBM footprint reserves before creating it.
  * No counterpart in the reference (its matrices live in host memory). */
 int ogsx_pattern_footprint(const ogsx_pattern* p, int precision, int64_t* bytes);
+/* Frees the pattern.  Systems keep pointers into their pattern, so it must
+ * outlive them: while systems created on it exist the call fails with
+ * OGSX_ESTATE and frees nothing. */
 int ogsx_pattern_destroy(ogsx_pattern* p);
 
 /* ---- system: Linear_EQS + the element data it is assembled from -------- */

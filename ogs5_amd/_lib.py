@@ -208,8 +208,9 @@ class Pattern:
         return b.value
 
     def close(self):
+        """Frees the pattern; refused (OgsxError) while systems on it exist."""
         if self.h:
-            lib().ogsx_pattern_destroy(self.h)
+            check(lib().ogsx_pattern_destroy(self.h), "ogsx_pattern_destroy")
             self.h = None
 
     def __del__(self):

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -94,6 +95,9 @@ struct PatternLayout {
 struct ogsx_pattern {
     PatternHost h;
     PatternLayout L;
+    // systems created on this pattern and not yet destroyed: they keep
+    // pointers into h and L, so the pattern outlives them
+    mutable std::atomic<int> users{0};
 };
 
 template <typename T>
@@ -128,6 +132,7 @@ struct SolveCtx {
 
 struct ogsx_system {
     const PatternHost* pat = nullptr;
+    const ogsx_pattern* owner = nullptr;  // the pattern pat and lay point into
     int precision = OGSX_FP64;
     int n = 0;
     int64_t nnz = 0;
@@ -900,6 +905,11 @@ int ogsx_system_bytes(const ogsx_system* s, int64_t* bytes)
 
 int ogsx_pattern_destroy(ogsx_pattern* p)
 {
+    if (!p) return OGSX_OK;
+    const int users = p->users.load();
+    if (users > 0)
+        return fail(OGSX_ESTATE, "pattern still used by " + std::to_string(users) +
+                                     " system(s): destroy them first");
     delete p;
     return OGSX_OK;
 }
@@ -921,6 +931,7 @@ int ogsx_system_destroy(ogsx_system* s)
     if (s->irs_host) (void)hipHostFree(s->irs_host);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->owner) s->owner->users--;
     delete s;
     return OGSX_OK;
 }
@@ -944,6 +955,8 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
     auto* s = new (std::nothrow) ogsx_system();
     if (!s) return fail(OGSX_ENOMEM, "host allocation failed");
     s->pat = &p->h;
+    s->owner = p;
+    p->users++;
     s->precision = precision;
     s->n = (int)p->h.n;
     s->nnz = p->h.nnz;
@@ -994,7 +1007,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
             (int64_t)h.conn.size(), h.npe, s->conn, s->ptr, s->col, s->slot);
     const int nb = blocks_for(h.nnz, kBlock * 4);
     k_fill<double><<<nb, kBlock, 0, s->stream>>>(h.nnz, s->A, 0.0);
-    double* zv[] = {s->v64.x, s->v64.b, s->uold};
+    double* zv[] = {s->v64.x, s->v64.b, s->uold, s->ic};
     for (double* q : zv) k_fill<double><<<blocks_for(h.n, kBlock * 4), kBlock, 0, s->stream>>>(h.n, q, 0.0);
     k_fill<int8_t><<<blocks_for(h.n, kBlock * 4), kBlock, 0, s->stream>>>(h.n, s->bcflag, (int8_t)0);
     if (hipStreamSynchronize(s->stream) != hipSuccess || hipGetLastError() != hipSuccess)

@@ -16,6 +16,8 @@ from ogs5_amd import mesh as M
 from ogs5_amd import workloads as W
 from oracle import driver
 
+from closed_forms import bmskel_node1
+
 LIQ, HEAT = 1, 3
 
 
@@ -161,6 +163,71 @@ def test_time_discretisation_identities(ffi):
         np.testing.assert_allclose(p.element_system(0)[1].sum(), 2.0 * Mo.sum() / dt, rtol=1e-13)
 
 
+@pytest.mark.parametrize("theta", [1.0, 0.5, 0.25, 0.0])
+@pytest.mark.parametrize("proc", [LIQ, HEAT])
+def test_theta_split_pinned(ffi, proc, theta):
+    """The theta split itself, against closed-form element matrices:
+    LHS = M/dt + th*(K+A) and RHS = (M/dt - (1-th)*(K+A)) u_n, with
+    th = nls_relaxation for LIQUID_FLOW (0 taken as 1,
+    fem_ele_std.cpp:7715-7720, :7841-7880) and th = ls_theta for
+    HEAT_TRANSPORT (:8509, :8630-8694).  theta = 0.25 fails a swap of theta
+    and 1 - theta; theta = 1 fails explicit in place of implicit Euler."""
+    hx, hy, k, s, dt, area = 3.0, 2.0, 0.4, 3.0, 7.0, 1.3
+    m = M.quad_mesh(1, 1, hx, hy)
+    u = np.array([1.0, 2.0, -0.5, 0.25])
+    coords = np.stack([m.x, m.y, m.z], 1)[m.conn[0]]
+    Ke, Me = _box_exact(coords, (hx, hy), k, s)
+    Ke, Me = area * Ke, area * Me
+    kw = {}
+    Ae = np.zeros_like(Ke)
+    if proc == HEAT:
+        # constant velocity: A_ij = int N_i (c v . grad N_j), exact on the
+        # bilinear element as the 1-D tensor products of N and dN/dx
+        v, c = np.array([0.7, -0.3, 0.0]), 2.0
+        idx = np.rint((coords[:, :2] - coords[:, :2].min(0)) / (hx, hy)).astype(int)
+        N1 = [np.array([[2, 1], [1, 2]]) * h / 6 for h in (hx, hy)]  # int N_a N_b
+        D1 = [np.array([[-1, 1], [-1, 1]]) / 2.0 for _ in (hx, hy)]  # int N_a dN_b
+        for d in range(2):
+            term = np.ones((4, 4))
+            for e in range(2):
+                mat = D1[e] if e == d else N1[e]
+                term = term * mat[np.ix_(idx[:, e], idx[:, e])]
+            Ae += c * v[d] * term
+        Ae *= area
+        kw = dict(adv_coef=c, vel=np.repeat(v[:, None], 9, axis=1)[None])
+    p = _prob(ffi, m, process=proc, mass_coef=[s], cond=[k], dt=dt, theta=theta, u_old=u,
+              geo_area=area, **kw)
+    Mo, Lo, Ao, _ = p.element_matrices(0)
+    np.testing.assert_allclose(Ao, Ae, rtol=1e-12, atol=1e-13 * abs(Ke).max())
+    lhs, rhs = p.element_system(0)
+    ul = u[m.conn[0]]
+    th = 1.0 if (proc == LIQ and theta == 0.0) else theta
+    KA = Ke + Ae
+    scale = abs(Me / dt).max() + abs(KA).max()
+    np.testing.assert_allclose(lhs, Me / dt + th * KA, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_allclose(rhs, (Me / dt - (1.0 - th) * KA) @ ul, rtol=0,
+                               atol=1e-13 * scale * abs(ul).max())
+
+
+def test_implicit_euler_huge_step_is_the_steady_solve(ffi):
+    """theta = 1 with dt -> infinity: one time step from any u_n lands on the
+    steady solution of K u = f (dense solve of the closed-form system).  The
+    explicit step (theta and 1 - theta swapped) would return M/dt-weighted
+    u_n - K u_n instead."""
+    sc = W.quad_flow(6, 5, seed=3, n_steps=1)
+    sc.steps = [W.Step(1e14, 1.0)]
+    sc.u0 = np.random.default_rng(1).standard_normal(sc.n) * 5.0
+    r = driver.run_oracle(sc, ffi, method=2, symmetric=False, tol=1e-14)
+    steady = W.quad_flow(6, 5, seed=3, n_steps=1)
+    steady.steps = [W.Step(1.0, 1.0, True)]
+    rs = driver.run_oracle(steady, ffi, method=2, symmetric=False, tol=1e-14, keep_steps=True)
+    eq = rs["eqs"]
+    a, b = rs["systems"][0]
+    eq.a[:] = a
+    x_dense = np.linalg.solve(eq.dense(), b)
+    assert np.linalg.norm(r["x"] - x_dense) / np.linalg.norm(x_dense) < 1e-9
+
+
 def test_reference_dense_multi_golden(ffi):
     """tests/Matrix/testMatrix.cpp: [[1,2,3],[4,5,6],[7,8,9]]*[1,2,3] = [14,32,50];
     after m = 1; m*=10; m/=2; m+=5 every row of m*1 is 30."""
@@ -292,7 +359,8 @@ def test_bmskel_golden(ffi):
     sc = W.bmskel()
     r = driver.run_oracle(sc, ffi, method=2, symmetric=False)
     assert r["x"][0] == 274.0
-    assert np.isfinite(r["x"][1])
+    assert abs(r["x"][1] - bmskel_node1()) <= 1e-12 * abs(bmskel_node1())
+
 
 
 def test_write_bin_layout(ffi, tmp_path):
