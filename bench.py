@@ -73,7 +73,8 @@ def gpu_leg(sc, precision, steps, warmup):
     # event records add a few microseconds of idle time per launch, so they
     # stay out of the timed region)
     s.profile(True)
-    run_job(sc, system, precision)
+    steps_x = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=precision,
+                        system=system, ic_on_device=True, fetch_x=False, keep_steps=True)["steps"]
     kind = ox._lib.PROF_SPMV64 if precision == ox.FP64 else ox._lib.PROF_SPMV32
     ms, launches, bpl = s.profile_read(kind)
     ms64, l64, b64 = s.profile_read(ox._lib.PROF_SPMV64)
@@ -85,6 +86,7 @@ def gpu_leg(sc, precision, steps, warmup):
     k_ms, k_bytes = s.bench_spmv(precision == ox.FP32_IR, 50)
     solves = len(sc.steps) * steps
     last = infos[-1]
+    spmv_in_ms = ms / max(launches, 1)
     assert all(i["converged"] == 1 for job in infos for i in job), infos[-1]
     assert all(i["error"] <= sc.tol for job in infos for i in job)
     return dict(
@@ -96,8 +98,9 @@ def gpu_leg(sc, precision, steps, warmup):
         errors=[i["error"] for i in last],
         spmv=dict(avg_ms=k_ms, bytes_per_launch=k_bytes, gbs=k_bytes / k_ms / 1e6, launches=50,
                   kernel_alone=True),
-        spmv_in_solve=dict(avg_ms=ms / max(launches, 1), launches=launches, bytes_per_launch=bpl,
-                           gbs=bpl / (ms / max(launches, 1)) / 1e6 if launches else None,
+        spmv_in_solve=dict(avg_ms=spmv_in_ms, launches=launches, bytes_per_launch=bpl,
+                           gbs=bpl / spmv_in_ms / 1e6 if launches else None,
+                           frac=bpl / spmv_in_ms / 1e6 / HBM_PEAK_GBS if launches else None,
                            iterations=sum(i["iterations"] for i in infos[-1]),
                            untimed_job=True),
         spmv64=dict(avg_ms=ms64 / max(l64, 1), launches=l64, bytes_per_launch=b64,
@@ -105,44 +108,65 @@ def gpu_leg(sc, precision, steps, warmup):
         spmv_variant=s.spmv_variant(),
         spmv_format=s.spmv_format(precision == ox.FP32_IR),
         spmv_tuning=s.spmv_tuning(),
+        steps_x=steps_x,
+        system=system,
     )
 
 
-def cpu_baseline(iters_per_solve, n_target, nnz_target, e_target, n_solves):
-    """Serial oracle (the CPU restatement of the reference path) on a bounded
-    sample of the same workload: the 48^3 instance, full assembly and 150
-    Jacobi-CG iterations, timed; the per-element assembly time and the
-    per-nonzero iteration time are scaled to the 128^3 job with the
-    iteration counts the GPU solve needed (same algorithm, same tolerance)."""
+def error_vs_fp64_ref(sc, system, legs, tol_ref=1e-13):
+    """Relative L2 error of every step's solution of each leg against the
+    same job solved in FP64 to a tight residual (the device FP64 CG, whose
+    solutions match the oracle within 1e-10 at every size the oracle can
+    run).  Errors include what earlier steps pass on through u_old."""
+    import ogs5_amd as ox
     from ogs5_amd import workloads as W
+    ref = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=ox.FP64,
+                    system=system, ic_on_device=True, fetch_x=False, keep_steps=True,
+                    tol=tol_ref)
+    assert all(i["converged"] == 1 for i in ref["info"]), ref["info"]
+    out = dict(reference=f"FP64 Jacobi-CG job to {tol_ref:g} relative residual on the device",
+               reference_residuals=[i["error"] for i in ref["info"]])
+    for name, xs in legs.items():
+        out[name] = [float(np.linalg.norm(x - r) / np.linalg.norm(r))
+                     for x, r in zip(xs, ref["steps"])]
+    return out
+
+
+def cpu_baseline(sc, iters_per_solve):
+    """Serial oracle (the CPU restatement of the reference path) on a bounded
+    sample of the job itself: one full assembly of the 128^3 steady
+    operator, the reference's row-only Dirichlet elimination (Diagonize)
+    and 20 Jacobi-CG iterations on that matrix, timed; the job's CPU time is
+    taken as one assembly per solve plus the iteration time times the
+    iteration counts the GPU's FP64 solve needed (same algorithm, same
+    tolerance).  The CRS pattern is built before timing, as on the GPU."""
     from oracle import ffi
-    m = 48
-    sc = W.hex_flow(m, n_transient=0)
     ptr, col, diag = ffi.pattern(sc.n, sc.mesh.conn)
     eq = ffi.OracleEqs(ptr, col, diag)
+    st = sc.steps[0]
     prob = ffi.OracleProblem(sc.mesh, process=sc.process, mass_coef=sc.mass_coef, cond=sc.cond,
-                             dt=sc.steps[0].dt, theta=1.0, steady=True, u_old=sc.u0)
+                             dt=st.dt, theta=st.theta, steady=st.steady, u_old=sc.u0)
     t0 = time.perf_counter()
     eq.initialize()
     eq.assemble(prob)
+    eq.set_known_x(sc.bc_idx, sc.bc_val, symmetric=False)
     t_asm = time.perf_counter() - t0
-    # symmetric elimination (untimed) so that CG runs on an SPD system
-    eq.set_known_x(sc.bc_idx, sc.bc_val, symmetric=True)
-    eq.config(5, 1, 1e-30, 150)
+    its = 20
+    eq.config(5, 1, 1e-30, its)
     t0 = time.perf_counter()
     out = eq.solve()
     t_it = (time.perf_counter() - t0) / max(out["iterations"], 1)
-    nnz = int(ptr[-1])
-    per_elem = t_asm / sc.mesh.n_elem
-    per_nnz_iter = t_it / nnz
-    total = n_solves * per_elem * e_target + sum(iters_per_solve) * per_nnz_iter * nnz_target
-    return dict(value=n_solves * n_target / total / 1e6, unit="MDOF-solves/s", cores=1,
-                kind="port",
-                sample=(f"oracle (serial C) on the {m}^3 instance: assembly {t_asm:.2f} s for "
-                        f"{sc.mesh.n_elem} hexes, {out['iterations']} CG iterations at "
-                        f"{1e3 * t_it:.1f} ms; scaled per element and per nonzero to the 128^3 "
-                        f"job with the GPU's iteration counts {iters_per_solve}"),
-                assembly_s_per_elem=per_elem, iter_s_per_nnz=per_nnz_iter)
+    n_solves = len(iters_per_solve)
+    total = n_solves * t_asm + sum(iters_per_solve) * t_it
+    return dict(value=n_solves * sc.n / total / 1e6, unit="MDOF-solves/s", cores=1,
+                nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)), kind="port",
+                sample=(f"oracle (serial C, 1 thread) on the {sc.mesh.shape[0]}^3 job itself: "
+                        f"assembly + row-only Dirichlet {t_asm:.2f} s for {sc.mesh.n_elem} hexes, "
+                        f"{out['iterations']} Jacobi-CG iterations at {1e3 * t_it:.1f} ms on its "
+                        f"{int(ptr[-1])} nonzeros; job time = {n_solves} assemblies + "
+                        f"{sum(iters_per_solve)} iterations (the GPU FP64 solve's counts) = "
+                        f"{total:.1f} s"),
+                assembly_s=t_asm, iteration_s=t_it, job_s=total)
 
 
 def spmv_mode(size):
@@ -174,14 +198,15 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def ensemble_mode(count, steps):
+def ensemble_runs(count, steps, group_fp64=True):
     """Batched ensemble: `count` independent 2-D flow instances (quad side
     log-uniform 64-512, sigma_lnK 0.5-2.5), `steps` implicit steps each.
-    Three timed runs of the whole job: one group in FP32-IR and in FP64
-    (one launch per Krylov phase over all running instances, iteration
-    batches replayed as HIP graphs), and the plain method (ogsx_solve on one
-    instance at a time, FP64).  Untimed warm-up: every instance's first
-    step (SpMV format tuning, graph capture)."""
+    Timed runs of the whole job: one group in FP32-IR (and, with
+    group_fp64, in FP64) -- one launch per Krylov phase over all running
+    instances, iteration batches replayed as HIP graphs -- and the plain
+    method (ogsx_solve on one instance at a time, FP64).  Untimed warm-up:
+    every instance's first step (SpMV format tuning, graph capture).
+    Returns (runs, scenarios)."""
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
     ox.device_count()
@@ -189,35 +214,49 @@ def ensemble_mode(count, steps):
     total = sum(sc.n for sc in scs) * steps
     warm = [W.Scenario(**{**sc.__dict__, "steps": sc.steps[:1]}) for sc in scs]
     res = {}
-    for name, prec in (("group_fp32_ir", ox.FP32_IR), ("group_fp64", ox.FP64)):
+    legs = [("group_fp32_ir", ox.FP32_IR, True)]
+    legs += [("group_fp64", ox.FP64, True)] if group_fp64 else [("plain_fp64", ox.FP64, False)]
+    for name, prec, grouped in legs:
         systems = [W.make_system(sc, prec) for sc in scs]
-        W.run_group(warm, precision=prec, systems=systems)
-        for _, s in systems:
-            s.synchronize()
-        t0 = time.perf_counter()
-        out = W.run_group(scs, precision=prec, systems=systems)
-        for _, s in systems:
-            s.synchronize()
-        dt = time.perf_counter() - t0
-        its = [i["iterations"] for inf in out["info"] for i in inf]
-        assert all(i["converged"] == 1 and i["error"] <= 1e-10 for inf in out["info"] for i in inf)
-        res[name] = dict(seconds=dt, mdof_solves_per_s=total / dt / 1e6,
-                         iterations_max=max(its), iterations_mean=float(np.mean(its)))
-        _log(f"ensemble {name}: {dt:.2f} s, {total / dt / 1e6:.1f} MDOF-solves/s")
-        out["group"].close()
+        if grouped:
+            W.run_group(warm, precision=prec, systems=systems)
+            for _, s in systems:
+                s.synchronize()
+            t0 = time.perf_counter()
+            out = W.run_group(scs, precision=prec, systems=systems)
+            for _, s in systems:
+                s.synchronize()
+            dt = time.perf_counter() - t0
+            its = [i["iterations"] for inf in out["info"] for i in inf]
+            assert all(i["converged"] == 1 and i["error"] <= 1e-10
+                       for inf in out["info"] for i in inf)
+            res[name] = dict(seconds=dt, mdof_solves_per_s=total / dt / 1e6,
+                             iterations_max=max(its), iterations_mean=float(np.mean(its)))
+            _log(f"ensemble {name}: {dt:.2f} s, {total / dt / 1e6:.1f} MDOF-solves/s")
+            out["group"].close()
         if prec == ox.FP64:
             for sc, sysw in zip(warm, systems):
                 W.run_gpu(sc, precision=ox.FP64, system=sysw)
+            for _, s in systems:
+                s.synchronize()
             t0 = time.perf_counter()
             its = []
             for sc, sysw in zip(scs, systems):
                 r = W.run_gpu(sc, precision=ox.FP64, system=sysw)
                 its += [i["iterations"] for i in r["info"]]
+                assert all(i["converged"] == 1 and i["error"] <= 1e-10 for i in r["info"])
             dt = time.perf_counter() - t0
             res["plain_fp64"] = dict(seconds=dt, mdof_solves_per_s=total / dt / 1e6,
                                      iterations_max=max(its), iterations_mean=float(np.mean(its)))
             _log(f"ensemble plain_fp64: {dt:.2f} s, {total / dt / 1e6:.1f} MDOF-solves/s")
+        for _, s in systems:
+            s.close()
         del systems
+    return res, scs
+
+
+def ensemble_mode(count, steps):
+    res, scs = ensemble_runs(count, steps)
     best = res["group_fp32_ir"]
     print(json.dumps({
         "metric": "MDOF-solves/s to 1e-10 FP64 resid",
@@ -549,6 +588,8 @@ def main():
     ap.add_argument("--requests2", type=int, default=150,
                     help="requests of the second served stream (--mode serve), at --rate2")
     ap.add_argument("--rate2", type=float, default=15.0)
+    ap.add_argument("--ensemble-instances", type=int, default=256,
+                    help="instances of the ensemble comparison in the full mode (0: none)")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests of the short served run in the full mode (0: none)")
     args = ap.parse_args()
@@ -580,6 +621,12 @@ def main():
     sc = W.hex_flow(args.size)
     fp64 = gpu_leg(sc, ox.FP64, args.steps, args.warmup)
     ir = gpu_leg(sc, ox.FP32_IR, args.steps, args.warmup)
+    # error of every step's solution of both solvers against the job solved
+    # tight in FP64 (untimed)
+    errors = error_vs_fp64_ref(sc, fp64["system"], {"fp64": fp64.pop("steps_x"),
+                                                    "fp32_ir": ir.pop("steps_x")})
+    for leg in (ir, fp64):
+        leg.pop("system")[1].close()
     best = ir if ir["mdof_solves_per_s"] > fp64["mdof_solves_per_s"] else fp64
     best_name = "fp32_ir" if best is ir else "fp64"
     sp = best["spmv"]
@@ -588,7 +635,11 @@ def main():
                 frac=(sp["gbs"] / peak) if sp["gbs"] else None, traffic=None,
                 kernel=("k_cg_spmv<float>" if best is ir else "k_cg_spmv<double>"),
                 avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"],
-                format=best["spmv_format"])
+                format=best["spmv_format"],
+                # the same kernel inside the solve (events around every launch
+                # of one untimed job): p is rewritten between launches
+                frac_in_solve=best["spmv_in_solve"]["frac"],
+                avg_launch_ms_in_solve=best["spmv_in_solve"]["avg_ms"])
     # HBM bytes per dispatch of this kernel and format, from the FETCH_SIZE /
     # WRITE_SIZE passes over `bench.py --mode spmv` (profiles/spmv_hex128_traffic.md)
     tf = os.path.join(ROOT, "profiles", "traffic.json")
@@ -600,9 +651,6 @@ def main():
             roof["traffic_source"] = key
     copy_gbs = ox._lib.stream_copy_gbs(1 << 30, 10)
     roof["copy_kernel_gbs"] = copy_gbs
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(fp64["iterations"], sc.n, _nnz(sc), sc.mesh.n_elem, len(sc.steps))
     out = {
         "metric": "MDOF-solves/s to 1e-10 FP64 resid; SpMV HBM GB/s; served p50/p99 latency",
         "value": best["mdof_solves_per_s"],
@@ -620,7 +668,8 @@ def main():
                                "steps, Jacobi-PCG to 1e-10",
                    "dof": sc.n, "nnz": _nnz(sc), "solver": best_name},
         "roofline": roof,
-        "cpu_baseline": cpu,
+        "cpu_baseline": None,
+        "error_vs_fp64_ref": errors,
         "fp64": fp64,
         "fp32_ir": ir,
         "ir_speedup_vs_fp64": ir["mdof_solves_per_s"] / fp64["mdof_solves_per_s"],
@@ -637,6 +686,18 @@ def main():
                              mdof_solves_per_s=sv["mdof_solves_per_s"],
                              note="short served run (continuous batching, FP32-IR); "
                                   "bench.py --mode serve runs the full comparison")
+    if args.ensemble_instances > 0:
+        # the batched-ensemble configuration: the FP32-IR group against the
+        # plain FP64 solver (one instance at a time), both to 1e-10
+        runs, scs = ensemble_runs(args.ensemble_instances, args.ens_steps, group_fp64=False)
+        out["ensemble"] = dict(
+            workload=f"{args.ensemble_instances} 2-D flow instances, quad side log-uniform "
+                     f"64-512, {args.ens_steps} implicit steps each, Jacobi-PCG to 1e-10",
+            dof_total=sum(x.n for x in scs), runs=runs,
+            speedup_vs_plain_fp64=runs["group_fp32_ir"]["mdof_solves_per_s"] /
+            runs["plain_fp64"]["mdof_solves_per_s"])
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, fp64["iterations"])
     print(json.dumps(out))
 
 

@@ -220,6 +220,11 @@ typedef struct {
 #define OGSX_IR_RELIABLE 1 /* CG with reliable updates: one FP32 recurrence,
                               FP64 residual replacement whenever ||r|| drops by
                               delta; the search direction survives updates    */
+#define OGSX_IR_RELIABLE_SR 2 /* as OGSX_IR_RELIABLE, with the FP32 recurrence
+                              in single-reduction form (Chronopoulos-Gear: one
+                              reduction point, two kernels per iteration) on
+                              the symmetrically Jacobi-scaled FP32 matrix
+                              S A S, S = diag(1/sqrt|a_ii|); CG only          */
 
 typedef struct {
     int iterations;    /* Krylov iterations (inner iterations summed for IR)  */
@@ -302,11 +307,13 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
 #define OGSX_PROF_SPMV32 1  /* FP32 SpMV kernel (IR inner solve)             */
 #define OGSX_PROF_ASSEMBLY 2
 #define OGSX_PROF_VELOCITY 3 /* ogsx_gauss_velocity kernel (on the flow system) */
-/* SpMV storage/launch variant: 0 = autotune (time variants 1, 2, 3, 5, 6, 7,
- * 10, 11, 12, 13, 14, 17, 18, 19, 20 on this matrix at first use and keep the
- * fastest; all of them sum each row in
- * CRS order over rounded products, bit-identical to CSparseMatrix::multiVec,
- * so the choice never changes results); 1 = tiled CRS (blocks stream
+/* SpMV storage/launch variant: 0 = autotune (time variants 3, 5, 6, 7, 10,
+ * 11, 12, 13, 14, 17, 18, 19, 20 on this matrix at first use and keep the
+ * fastest; all of them sum each row in CRS order over rounded products,
+ * bit-identical to CSparseMatrix::multiVec, and hand each row to the same
+ * thread of the same block, so the dot products the solve fuses into its
+ * SpMV round identically too: the choice never changes results; the same
+ * holds for 9); 1 = tiled CRS (blocks stream
  * contiguous nonzero tiles through LDS); 2 = tiled CRS with 16-bit column
  * offsets per tile (when every tile's columns span < 65536); 3 = SELL-64
  * (a device copy of the CRS values in 64-row slices, column-major, one lane
@@ -323,9 +330,12 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * its +1 / +2 neighbours by lane shifts; zero entries where a row lacks a
  * column of its slice's stencil; when every slice needs at most 16 triples
  * and the stored entries are at most twice the nonzeros), 5 / 4 triples in
- * flight, non-temporal / cached; 4, 8, 16, 32,
- * 64 = vector CRS with that many lanes per row (lane-parallel partial sums:
- * a different rounding order).  The CRS arrays stay the system's matrix;
+ * flight, non-temporal / cached; 9 = plain CRS with one thread per row (no
+ * layout beyond the CRS arrays; the zero-allocation solve's format); 4, 8,
+ * 16, 32, 64 = vector CRS with that many lanes per row (lane-parallel
+ * partial sums: a different rounding order).  With 1, 2 and 4-64 the fused
+ * dot products of a solve accumulate rows in another order, so solves
+ * round differently (within the usual FP64 tolerance).  The CRS arrays stay the system's matrix;
  * SELL copies are refreshed after A changes. */
 int ogsx_set_spmv_variant(ogsx_system* s, int variant);
 int ogsx_get_spmv_variant(ogsx_system* s, int* variant);

@@ -38,7 +38,7 @@ class Elements(C.Structure):
     ]
 
 
-IR_RESTART, IR_RELIABLE = 0, 1
+IR_RESTART, IR_RELIABLE, IR_RELIABLE_SR = 0, 1, 2
 
 
 class SolverConfig(C.Structure):
@@ -167,7 +167,7 @@ def solver_config(method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
     if ir_mode is None:
         ir_mode = IR_RELIABLE if method == CG else IR_RESTART
     if inner_tol is None:
-        inner_tol = 0.1 if ir_mode == IR_RELIABLE else 1e-4
+        inner_tol = 0.1 if ir_mode in (IR_RELIABLE, IR_RELIABLE_SR) else 1e-4
     return SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer, ir_mode)
 
 
@@ -430,3 +430,4 @@ def solve_status(system: "System"):
     info = SolveInfo()
     check(lib().ogsx_solve_status(system.h, C.byref(a), C.byref(info)), "ogsx_solve_status")
     return bool(a.value), info.as_dict()
+

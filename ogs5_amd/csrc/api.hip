@@ -119,6 +119,7 @@ struct MemberSet;
 struct SolveCtx {
     int active = 0;  // solve in progress
     int method = OGSX_CG;
+    int kmethod = OGSX_CG;  // kernel set (kernel_method)
     ogsx_solver_config cfg{};
     bool ir = false;  // FP32 recurrence inside FP64 refinement
     int total = 0, updates = 0;
@@ -170,6 +171,9 @@ struct ogsx_system {
     float* sval32 = nullptr;
     uint64_t a_version = 1, sval64_version = 0, sval32_version = 0, a32_version = 0;
     int sval64_layout = 0, sval32_layout = 0;
+    // FP32 copies (A32, sval32) hold A (0) or S A S with S from the Jacobi
+    // diagonal (1) or S = I (2)
+    int a32_scaled = 0, sval32_scaled = 0;
     hipStream_t stream = nullptr;
     // pattern on device
     int *ptr = nullptr, *col = nullptr, *diagpos = nullptr;
@@ -216,6 +220,9 @@ struct ogsx_system {
     size_t vel_cap = 0;
     int process = 0, dim = 0, ngp = 0, cond_stride = 1;
     double tuf = 1, mu = 1, geo_area = 1, adv_coef = 0, rho_g = 0;
+    bool tensor_sym = true;   // every element tensor symmetric
+    bool a_symmetric = true;  // A is symmetric by construction (symmetric
+                              // tensors, no advection, no row-only Dirichlet)
     double *ex = nullptr, *ey = nullptr, *ez = nullptr, *mass_coef = nullptr, *cond = nullptr,
            *vel = nullptr, *tabN = nullptr, *tabdN = nullptr, *tabW = nullptr;
     double* gvel = nullptr;  // Gauss-point velocity computed for the host only
@@ -296,7 +303,7 @@ bool valid_fmt(const ogsx_system* s, int f)
         return s->sell16;
     if (is_pat(f)) return s->pat_ok;
     if (is_tri(f)) return s->tri_ok;
-    return f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
+    return f == kFmtCrs1 || f == 4 || f == 8 || f == 16 || f == 32 || f == 64;
 }
 
 // Formats that read the row tiles or the 32-bit SELL columns.
@@ -435,22 +442,26 @@ int sync_sell(ogsx_system* s, bool want64, bool want32)
     const Grp g1{s->dev, nullptr, nullptr};
     const int lay64 = is_tri(s->fmt64) ? 1 : 0, lay32 = is_tri(s->fmt32) ? 1 : 0;
     const bool stale64 = want64 && (s->sval64_version != s->a_version || s->sval64_layout != lay64);
-    const bool stale32 =
-        want32 && s->sval32 && (s->sval32_version != s->a_version || s->sval32_layout != lay32);
+    const bool stale32 = want32 && s->sval32 &&
+                         (s->sval32_version != s->a_version || s->sval32_layout != lay32 ||
+                          s->sval32_scaled != 0);
     if (stale64 && stale32 && lay64 == lay32) {
         k_to_sell_both_g<<<s->G, kBlock, 0, s->stream>>>(g1, lay64);
         s->sval64_version = s->sval32_version = s->a_version;
         s->sval64_layout = s->sval32_layout = lay64;
+        s->sval32_scaled = 0;
     }
     if (want64 && (s->sval64_version != s->a_version || s->sval64_layout != lay64)) {
         k_to_sell_g<double><<<s->G, kBlock, 0, s->stream>>>(g1, lay64);
         s->sval64_version = s->a_version;
         s->sval64_layout = lay64;
     }
-    if (want32 && s->sval32 && (s->sval32_version != s->a_version || s->sval32_layout != lay32)) {
+    if (want32 && s->sval32 &&
+        (s->sval32_version != s->a_version || s->sval32_layout != lay32 || s->sval32_scaled != 0)) {
         k_to_sell_g<float><<<s->G, kBlock, 0, s->stream>>>(g1, lay32);
         s->sval32_version = s->a_version;
         s->sval32_layout = lay32;
+        s->sval32_scaled = 0;
     }
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
@@ -692,6 +703,7 @@ int prof_flush(ogsx_system* s)
         case kFmtSellPat7C: { constexpr int L = kFmtSellPat7C; CALL; } break; \
         case kFmtTri5: { constexpr int L = kFmtTri5; CALL; } break;           \
         case kFmtTri4C: { constexpr int L = kFmtTri4C; CALL; } break;         \
+        case kFmtCrs1: { constexpr int L = kFmtCrs1; CALL; } break;           \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -714,25 +726,31 @@ void launch_solve_spmv(ogsx_system* s, int fmt)
                           Grp{s->dev, nullptr, nullptr})));
 }
 
-// Autotuner: times the solve's SpMV kernel in every format that sums each
-// row in CRS order (tiled with 16/32-bit columns, SELL-64 variants) on this
-// matrix and keeps the fastest.  All candidates give bit-identical results,
-// so the choice (which depends on timing) never changes a result.  The
-// Krylov vectors are scratch at this point; p is set to 1.
+// Autotuner: times the solve's SpMV kernel in every slice-based format
+// (SELL-64 variants, stencil dictionary, slice triples) on this matrix and
+// keeps the fastest.  All candidates sum each row in CRS order and hand it
+// to the same thread of the same block, so the row sums and the dot
+// products fused into the solve kernels are bit-identical between them:
+// the choice (which depends on timing) never changes a result.  The Krylov
+// vectors are scratch at this point; p is set to 1.
 template <typename VT, typename T>
 int tune_one(ogsx_system* s, int* fmt_out)
 {
+    // the slice-based formats only: they hand every row to the same thread
+    // of the same block, so the dot products fused into the solve kernels
+    // round identically with each of them (the tiled formats, selectable
+    // by hand, accumulate rows in another order)
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
                          kFmtSell16X2,  kFmtSell16U7, kFmtSellPat8, kFmtSellPat7,
                          kFmtSellPat8C, kFmtSellPat7C, kFmtTri5,    kFmtTri4C,
-                         kFmtSell32,    kFmtTile16,   kFmtTile32};
+                         kFmtSell32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     T* p = sizeof(T) == 8 ? (T*)s->v64.p : (T*)s->v32.p;
     k_fill<T><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, p, (T)1);
     k_set_done<<<1, 64, 0, s->stream>>>(s->st, 0);
-    int best = kFmtTile32;
+    int best = kFmtSell32;
     float best_ms = 1e30f;
     const int reps = 4;
     for (int f : cands) {
@@ -1150,6 +1168,16 @@ int ogsx_system_set_elements(ogsx_system* s, const ogsx_elements* e)
     HIP_TRY(up(s->tabN, t.N.data(), 8 * t.N.size()));
     HIP_TRY(up(s->tabdN, t.dN.data(), 8 * t.dN.size()));
     HIP_TRY(up(s->tabW, t.w.data(), 8 * t.w.size()));
+    // a symmetric conductivity/permeability tensor (and no advection) makes
+    // the assembled operator symmetric
+    s->tensor_sym = true;
+    if (e->cond_stride == ed * ed && ed > 1)
+        for (int64_t el = 0; el < h.n_elem && s->tensor_sym; el++) {
+            const double* T = e->cond + (size_t)el * ed * ed;
+            for (int i = 0; i < ed; i++)
+                for (int j = i + 1; j < ed; j++)
+                    if (T[i * ed + j] != T[j * ed + i]) s->tensor_sym = false;
+        }
     s->use_vel = (e->vel != nullptr) && e->process == OGSX_HEAT_TRANSPORT;
     if (s->use_vel) {
         const size_t need = (size_t)h.n_elem * 3 * t.ngp;
@@ -1337,6 +1365,7 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
     if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
     if (!s->elements_set) return fail(OGSX_ESTATE, "ogsx_system_set_elements not called");
     if (!(dt > 0.0)) return fail(OGSX_EINVAL, "dt must be positive");
+    s->a_symmetric = s->tensor_sym && !(s->process == OGSX_HEAT_TRANSPORT && s->use_vel);
     AsmArgs a;
     std::memset(&a, 0, sizeof(a));
     a.conn = s->conn;
@@ -1537,6 +1566,7 @@ int ogsx_set_dirichlet(ogsx_system* s, int64_t n, const int64_t* idx, const doub
         s->last_bc_val.assign(val, val + n);
     }
     const int m = s->bc_m;
+    if (mode == OGSX_DIRICHLET_ROW) s->a_symmetric = false;
     if (mode == OGSX_DIRICHLET_ROW) {
         k_bc_row<<<blocks_for(m, kBlock), kBlock, 0, s->stream>>>(m, s->bc_idx_d, s->bc_val_d, s->ptr,
                                                                   s->col, s->diagpos, s->A, s->v64.b);
@@ -1755,9 +1785,9 @@ int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
     if (!g || !cfg || count < 0 || (count > 0 && !systems)) return fail(OGSX_EINVAL, "bad argument");
     OK_TRY(check_config(cfg));
     MemberSet* m = g->set;
+    int kmethod = -1;
     for (ogsx_system* s : m->mem)
-        if (s->sc.active && s->sc.method != cfg->method)
-            return fail(OGSX_EINVAL, "solves running together in a group must share the method");
+        if (s->sc.active) kmethod = s->sc.kmethod;
     std::vector<int> which;
     for (int k = 0; k < count; k++) {
         const int i = member_index(g, systems[k]);
@@ -1765,6 +1795,11 @@ int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
         if (systems[k]->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
         if (std::find(which.begin(), which.end(), i) != which.end())
             return fail(OGSX_EINVAL, "system listed twice");
+        const int km = kernel_method(cfg, systems[k]);
+        if (kmethod >= 0 && km != kmethod)
+            return fail(OGSX_EINVAL, "solves running together in a group must share the method "
+                                     "(and the form of the FP32 recurrence)");
+        kmethod = km;
         which.push_back(i);
     }
     for (int i : which) OK_TRY(autotune(m->mem[i]));
@@ -1930,9 +1965,9 @@ int ogsx_set_spmv_variant(ogsx_system* s, int lanes_per_row)
     if (!valid_fmt(s, lanes_per_row))
         return fail(OGSX_EINVAL, "SpMV variant must be 1/2 (tiled CRS, 32/16-bit columns), "
                                  "3/5/6/7/10/11/12 (SELL-64), 13/14/17/18 (SELL-64 with a stencil "
-                                 "dictionary), 19/20 (SELL-64 slice triples), or 4, 8, 16, 32, 64 "
-                                 "lanes per row (16-bit, dictionary and triple forms only when the "
-                                 "pattern allows)");
+                                 "dictionary), 19/20 (SELL-64 slice triples), 9 (plain CRS, one "
+                                 "thread per row), or 4, 8, 16, 32, 64 lanes per row (16-bit, "
+                                 "dictionary and triple forms only when the pattern allows)");
     if (needs_extra_layouts(lanes_per_row) && !s->extra_layouts) {
         OK_TRY(alloc_extra_layouts(s));
         // the matrix views of the device descriptor now hold the new arrays
@@ -1997,8 +2032,10 @@ int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* 
         for (int r = 0; r < reps; r++) launch_solve_spmv<double, double>(s, s->fmt64);
         (void)hipEventRecord(e1, s->stream);
     } else {
-        if (!is_sell(s->fmt32))
+        if (!is_sell(s->fmt32)) {
             k_cast_f32_g<<<s->G, kBlock, 0, s->stream>>>(Grp{s->dev, nullptr, nullptr});
+            s->a32_scaled = 0;
+        }
         OK_TRY(sync_sell(s, false, is_sell(s->fmt32)));
         k_fill<float><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v32.p, 1.0f);
         k_set_done<<<1, 64, 0, s->stream>>>(s->st, 0);

@@ -271,6 +271,8 @@ constexpr int kPatLds = 4096;             // dictionary entries staged in LDS
 constexpr int kFmtTri5 = 19;              // SELL-64 slice triples, 5 in flight, NT
 constexpr int kFmtTri4C = 20;             // ... 4 in flight, cached loads
 constexpr int kTriMax = 16;               // triples per slice stencil
+constexpr int kFmtCrs1 = 9;               // plain CRS, one thread per row (needs only
+                                          // ptr, col, val)
 
 // Tiled CRS SpMV: a block streams the contiguous nonzeros of one tile of
 // rows (<= kTileNnz nonzeros, <= kBlock rows) with kTileU independent
@@ -623,8 +625,33 @@ __device__ __forceinline__ void spmv_sell_x2(const GMat<VT>& m, const OGSX_GL XT
     }
 }
 
-// Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled or
-// SELL-64 format.
+// Plain CRS with one thread per row: each row summed left to right over
+// rounded products (CSparseMatrix::multiVec's order,
+// FEM/matrix_class.cpp:1527-1538).  Needs no layout beyond ptr, col and
+// val, so it serves matrices the library did not build (the zero-allocation
+// solve); the tuned formats stream faster.  Rows map to blocks and threads
+// as in the SELL-64 formats -- block b takes the rows of its chunk of 64-row
+// slices, thread t the rows R0 + t + 256 j -- so the fused dot products of
+// the solve kernels (accumulated per thread in that order, then reduced in
+// a fixed tree) round exactly as with every SELL-64 format.
+template <typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_crs1(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                          int G, int b, Epi&& epi)
+{
+#pragma clang fp contract(off)
+    int sb, se;
+    chunk_of((m.n + kSellC - 1) / kSellC, G, b, sb, se);
+    const int beg = sb * kSellC, end = min(se * kSellC, m.n);
+    for (int r = beg + (int)threadIdx.x; r < end; r += kBlock) {
+        const int k1 = m.ptr[r + 1];
+        ACC s = 0;
+        for (int k = m.ptr[r]; k < k1; k++) s += (ACC)m.val[k] * (ACC)x[m.col[k]];
+        epi(r, s);
+    }
+}
+
+// Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled,
+// SELL-64 or plain CRS format.
 template <int FMT, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
                                           int G, int b, Epi&& epi)
@@ -659,6 +686,8 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
         spmv_tri<5, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtTri4C)
         spmv_tri<4, false, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtCrs1)
+        spmv_crs1<VT, XT, ACC>(m, x, G, b, epi);
     else
         spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, G, b, epi);
 }
@@ -1199,7 +1228,8 @@ struct KrylovParams {
     int ir_max_iter;
     int ir_max_outer;     // restart: refinement steps
     int ir_reliable;      // 1 reliable-update CG, 0 restarted refinement
-    int pad;
+    int ir_scaled;        // 1: the FP32 recurrence runs on S A S with
+                          // S = diag(1/sqrt|a_ii|) (single-reduction CG)
 };
 
 // Refinement state of an IR solve, advanced by k_ir_decide.
@@ -1398,6 +1428,7 @@ __global__ void k_cg_init_final(Grp g, int want)
     const KrylovParams P = *S.prm;  // one block per system: a generic load is fine
     const double bnorm_new = sqrt(s[0]);
     st->iter = 0;
+    st->sr_it = 0;
     st->max_iter = P.max_iter;
     st->done = 0;
     st->pend = 0;
@@ -1874,7 +1905,8 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
 
 // ---- FP32-inside-FP64 iterative refinement helpers
 
-// r64 = b - A x in FP64; partials (b,b), (r,r)
+// r64 = b - A x in FP64; partials (b,b), (r,r) and (S r, S r) -- the norm
+// of the residual the scaled FP32 recurrence starts from (= (r,r) unscaled)
 template <int FMT>
 __global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
 {
@@ -1883,16 +1915,20 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Grp g)
     const GMat<double> m = gmat(S.m64);
     const GVec<double> v = gvec(S.v64);
     OGSX_GL double* r = gl(S.r64);
+    const bool scaled = gl(S.prm)->ir_scaled;
+    const OGSX_GL float* sc = gl(S.v32.d);
     {
         const int b = c.b;
-        double acc[2] = {0.0, 0.0};
+        double acc[3] = {0.0, 0.0, 0.0};
         spmv_rows<FMT, double, double, double>(m, v.x, c.G, b, [&](int i, double ax) {
             const double ri = v.b[i] - ax;
             r[i] = ri;
             acc[0] += v.b[i] * v.b[i];
             acc[1] += ri * ri;
+            const double si = scaled ? (double)sc[i] * ri : ri;
+            acc[2] += si * si;
         });
-        store_partials<2>(acc, gl(S.part), c.G, b);
+        store_partials<3>(acc, gl(S.part), c.G, b);
     }
 }
 
@@ -1906,7 +1942,13 @@ __global__ __launch_bounds__(kBlock) void k_ir_fold(Grp g)
     const double u = gl(S.prm)->unscale;
     OGSX_GL double* x = gl(S.v64.x);
     const OGSX_GL float* x32 = gl(S.v32.x);
-    vec_vblocks(c, S.n, [&](int i) { x[i] += u * (double)x32[i]; });
+    if (gl(S.prm)->ir_scaled) {
+        // the scaled recurrence solved (S A S) y = S r: the correction is S y
+        const OGSX_GL float* sc = gl(S.v32.d);
+        vec_vblocks(c, S.n, [&](int i) { x[i] += u * ((double)sc[i] * (double)x32[i]); });
+    } else {
+        vec_vblocks(c, S.n, [&](int i) { x[i] += u * (double)x32[i]; });
+    }
 }
 
 // One block per listed system: after k_ir_residual, the refinement step.
@@ -1919,8 +1961,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_fold(Grp g)
 __global__ void k_ir_decide(Grp g)
 {
     const SysDev& S = *sys_of_block(g);
-    double s[2];
-    reduce_partials<2>(gl(S.part), S.G, s);
+    double s[3];
+    reduce_partials<3>(gl(S.part), S.G, s);
     if (threadIdx.x != 0) return;
     OGSX_GL IRState& R = *gl(S.irs);
     OGSX_GL KrylovParams& P = *gl(S.prm);
@@ -1932,6 +1974,7 @@ __global__ void k_ir_decide(Grp g)
         if (st.done == 0) st.done = 4;
     };
     const double rn = sqrt(s[1]);
+    const double rs = sqrt(s[2]);  // ||S r||, = rn without scaling
     int inner_status = 0, inner_iters = 0;
     if (!R.started) {
         R.bnorm = sqrt(s[0]);
@@ -1948,10 +1991,12 @@ __global__ void k_ir_decide(Grp g)
             return;
         }
     } else {
-        // st.done == 0 with pend set: BiCGStab converged on s
-        inner_iters = st.iter;
+        // st.done == 0 with pend set: BiCGStab converged on s; the
+        // single-reduction CG counts its last update in sr_it
+        const int its = max(st.iter, st.sr_it);
+        inner_iters = its;
         inner_status = st.done == 2 ? 2 : (st.done == 3 || st.done == 6) ? 3 : 1;
-        R.total = P.ir_reliable ? st.iter : R.total + max(st.iter, 1);
+        R.total = P.ir_reliable ? its : R.total + max(its, 1);
         R.updates++;
     }
     R.error = rn / R.bnorm;
@@ -1969,15 +2014,18 @@ __global__ void k_ir_decide(Grp g)
         return;
     }
     if (P.ir_reliable && R.started) {
-        R.maxr = rn * P.scale;
+        // the recurrence resumes from r32 = scale S r64 (||r32|| = rs scale);
+        // its stopping target is the FP64 one, mapped by ||S r|| / ||r||
+        R.maxr = rs * P.scale;
         P.upd_thresh = P.ir_delta * R.maxr;
+        P.tol = P.ir_tol * R.bnorm * P.scale * (rs / rn);
         R.action = 2;
     } else {
-        P.scale = 1.0 / rn;
-        P.unscale = rn;
+        P.scale = 1.0 / rs;
+        P.unscale = rs;
         P.bnorm_prev = 0.0;
         if (P.ir_reliable) {
-            P.tol = P.ir_tol * R.bnorm * P.scale;  // the FP64 target in scaled units
+            P.tol = P.ir_tol * R.bnorm / rn;  // the FP64 target in units of ||r32|| = 1
             R.maxr = 1.0;
             P.upd_thresh = P.ir_delta;
         } else {
@@ -2002,10 +2050,281 @@ __global__ __launch_bounds__(kBlock) void k_ir_prepare(Grp g)
     OGSX_GL float* dst = gl(act == 1 ? S.v32.b : S.v32.r);
     OGSX_GL float* x32 = gl(S.v32.x);
     const OGSX_GL double* r64 = gl(S.r64);
+    if (gl(S.prm)->ir_scaled) {
+        const OGSX_GL float* sc = gl(S.v32.d);
+        vec_vblocks(c, S.n, [&](int i) {
+            dst[i] = (float)(r64[i] * scale * (double)sc[i]);
+            x32[i] = 0.0f;
+        });
+    } else {
+        vec_vblocks(c, S.n, [&](int i) {
+            dst[i] = (float)(r64[i] * scale);
+            x32[i] = 0.0f;
+        });
+    }
+}
+
+// ---- FP32 inner CG with one reduction point per iteration
+// (OGSX_IR_RELIABLE_SR): the Chronopoulos-Gear recurrence (J. Comput. Appl.
+// Math. 25 (1989) 153-168) on the symmetrically Jacobi-scaled FP32 matrix
+// S A S, S = diag(1/sqrt|a_ii|) -- in exact arithmetic Jacobi-PCG, with no
+// preconditioner left in the loop.  Per iteration two kernels:
+//   K1  w = (SAS) r; partials (r,w), (r,r)
+//   K2  gamma = (r,r), delta = (r,w); stopping test; beta = gamma/gamma_old,
+//       alpha = gamma/(delta - beta gamma/alpha_old) (beta = 0, alpha =
+//       gamma/delta at the first iteration); p = r + beta p, s = w + beta s,
+//       x += alpha p, r -= alpha s
+// w lives in the z vector.  Only the FP32 recurrence inside FP64 iterative
+// refinement uses it; the FP64 solver keeps the reference's recurrence.
+
+// S = diag(1/sqrt|a_ii|) into the FP32 diagonal vector (1 where |a_ii| <
+// DBL_MIN, and everywhere without a preconditioner).
+__global__ __launch_bounds__(kBlock) void k_diag_scale(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    OGSX_GL float* d = gl(S.v32.d);
+    const OGSX_GL int* dp = gl(S.diagpos);
+    const OGSX_GL double* A = gl(S.A);
+    const int jac = gl(S.prm)->jacobi;
     vec_vblocks(c, S.n, [&](int i) {
-        dst[i] = (float)(r64[i] * scale);
-        x32[i] = 0.0f;
+        double v = (jac && dp[i] >= 0) ? fabs(A[dp[i]]) : 1.0;
+        if (v < DBL_MIN) v = 1.0;
+        d[i] = (float)(1.0 / sqrt(v));
     });
+}
+
+// A32 = S A S (CRS order), from the FP64 values and the FP32 scale vector.
+__global__ __launch_bounds__(kBlock) void k_scale_a32(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    const GMat<double> m = gmat(S.m64);
+    OGSX_GL float* A32 = gl(S.A32);
+    const OGSX_GL float* sc = gl(S.v32.d);
+    vec_vblocks(c, S.n, [&](int i) {
+        const double si = (double)sc[i];
+        const int k1 = m.ptr[i + 1];
+        for (int k = m.ptr[i]; k < k1; k++)
+            A32[k] = (float)(m.val[k] * si * (double)sc[m.col[k]]);
+    });
+}
+
+// FP32 SELL / slice-triple values from A32 (tri = 1: the triple layout).
+__global__ __launch_bounds__(kBlock) void k_sell32_from_a32(Grp g, int tri)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    OGSX_GL float* sv = gl(S.sval32);
+    const OGSX_GL int* src = gl(tri ? S.tri_src : S.sell_src);
+    const OGSX_GL float* A32 = gl(S.A32);
+    vec_vblocks(c, tri ? S.npad3 : S.npad, [&](int p) {
+        const int k = src[p];
+        sv[p] = k >= 0 ? A32[k] : 0.0f;
+    });
+}
+
+// New recurrence: r = b - (SAS) x; partials (b,b), (r,r), (r,r) for
+// k_cg_init_final.
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_init(Grp g, int want)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    if (skip_action(S, want)) return;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    const GVec<T> v = gvec(kv_of<T>(S));
+    double acc[3] = {0.0, 0.0, 0.0};
+    spmv_rows<FMT, VT, T, T>(m, v.x, c.G, c.b, [&](int i, T ax) {
+        const T ri = v.b[i] - ax;
+        v.r[i] = ri;
+        acc[0] += (double)v.b[i] * (double)v.b[i];
+        acc[1] += (double)ri * (double)ri;
+        acc[2] += (double)ri * (double)ri;
+    });
+    store_partials<3>(acc, gl(S.part), c.G, c.b);
+}
+
+// K1: w = (SAS) r; partials (r,w), (r,r).  Block 0 takes over the scalars
+// the last update left (sr_*).  resume = 1: the first SpMV after a reliable
+// update, run for the systems whose refinement step chose to resume.
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_spmv(Grp g, int resume)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    OGSX_GL KrylovState* st = gl(S.st);
+    if (resume ? skip_action(S, 2) : st->done != 0) return;
+    const GMat<VT> m = gmat(mv_of<VT>(S));
+    const GVec<T> v = gvec(kv_of<T>(S));
+    double acc[2] = {0.0, 0.0};
+    spmv_rows<FMT, VT, T, T>(m, v.r, c.G, c.b, [&](int i, T ax) {
+        v.z[i] = ax;
+        const T ri = v.r[i];
+        acc[0] += (double)ri * (double)ax;
+        acc[1] += (double)ri * (double)ri;
+    });
+    store_partials<2>(acc, gl(S.part), c.G, c.b);
+    if (c.b == 0 && threadIdx.x == 0) {
+        st->rr_old = st->sr_gam;
+        st->alpha = st->sr_alp;
+        st->iter = st->sr_it;
+    }
+}
+
+// K2: stopping test on the current r, then the update.  With reliable
+// updates (upd_thresh > 0) the recurrence stops with done = 5 once ||r|| <
+// upd_thresh, before updating, leaving p and s for the resumption.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_update(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    OGSX_GL KrylovState* st = gl(S.st);
+    if (st->done) return;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    OGSX_GL T* __restrict__ x = v.x;
+    OGSX_GL T* __restrict__ r = v.r;
+    OGSX_GL T* __restrict__ p = v.p;
+    OGSX_GL T* __restrict__ sv = v.s;
+    const OGSX_GL T* __restrict__ w = v.z;
+    int beg, end;
+    chunk_of(S.n, c.G, c.b, beg, end);
+    constexpr int U = 4;
+    int i = beg + threadIdx.x;
+    // the first batch of the stream does not depend on the scalars: its
+    // loads are issued before the partials are reduced
+    T xi[U], pi[U], ri[U], si[U], wi[U];
+    const bool first = i + (U - 1) * kBlock < end;
+    if (first) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            xi[u] = x[k];
+            pi[u] = p[k];
+            ri[u] = r[k];
+            si[u] = sv[k];
+            wi[u] = w[k];
+        }
+    }
+    double red[2];
+    reduce_partials<2>(gl(S.part), c.G, red);
+    const double delta = red[0], gamma = red[1];
+    const double rnorm = sqrt(gamma);
+    const double error = rnorm / st->bnorm;
+    const double tol = gl(S.prm)->tol;
+    if (error < tol || rnorm < gl(S.prm)->upd_thresh) {
+        if (c.b == 0 && threadIdx.x == 0) {
+            st->error = error;
+            st->done = error < tol ? 1 : 5;
+        }
+        return;
+    }
+    const int it = st->iter;
+    const bool start = it == 0;
+    double alpha_d, beta_d = 0.0;
+    if (start) {
+        alpha_d = gamma / delta;
+    } else {
+        beta_d = gamma / st->rr_old;
+        alpha_d = gamma / (delta - beta_d * gamma / st->alpha);
+    }
+    const T alpha = (T)alpha_d, beta = (T)beta_d;
+    for (bool loaded = first; i + (U - 1) * kBlock < end; i += U * kBlock, loaded = false) {
+        if (!loaded) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = i + u * kBlock;
+                xi[u] = x[k];
+                pi[u] = p[k];
+                ri[u] = r[k];
+                si[u] = sv[k];
+                wi[u] = w[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = i + u * kBlock;
+            const T pn = start ? ri[u] : ri[u] + beta * pi[u];
+            const T sn = start ? wi[u] : wi[u] + beta * si[u];
+            p[k] = pn;
+            sv[k] = sn;
+            x[k] = xi[u] + alpha * pn;
+            r[k] = ri[u] - alpha * sn;
+        }
+    }
+    for (; i < end; i += kBlock) {
+        const T pn = start ? r[i] : r[i] + beta * p[i];
+        const T sn = start ? w[i] : w[i] + beta * sv[i];
+        p[i] = pn;
+        sv[i] = sn;
+        x[i] += alpha * pn;
+        r[i] -= alpha * sn;
+    }
+    if (c.b == 0 && threadIdx.x == 0) {
+        st->sr_gam = gamma;
+        st->sr_alp = alpha_d;
+        st->sr_it = it + 1;
+        st->error = error;
+        if (it + 1 >= st->max_iter) st->done = 2;
+    }
+}
+
+// Resumption after a reliable update (r replaced by the scaled true
+// residual, x = 0, p and s kept), after k_cg1_spmv(resume): the direction
+// step with beta = (r,r)/gamma_old, then the partial (p, s) -- the
+// recurrence's (p, A p) identity assumes the residuals it produced itself,
+// so the first step after a replacement takes alpha from the explicit dot.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_resume(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    if (skip_action(S, 2)) return;
+    OGSX_GL KrylovState* st = gl(S.st);
+    double red[2];
+    reduce_partials<2>(gl(S.part), c.G, red);
+    const double gamma = red[1];
+    const T beta = (T)(gamma / st->rr_old);
+    const GVec<T> v = gvec(kv_of<T>(S));
+    double acc[1] = {0.0};
+    vec_chunk(S.n, c.G, c.b, [&](int i) {
+        const T pn = v.r[i] + beta * v.p[i];
+        const T sn = v.z[i] + beta * v.s[i];
+        v.p[i] = pn;
+        v.s[i] = sn;
+        acc[0] += (double)pn * (double)sn;
+    });
+    store_partials<1>(acc, gl(S.part) + 4 * (size_t)c.G, c.G, c.b);
+    if (c.b == 0 && threadIdx.x == 0) st->rr = gamma;
+}
+
+// ... and its update: alpha = gamma/(p,s), x += alpha p, r -= alpha s; the
+// recurrence runs again.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_resume_step(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    if (skip_action(S, 2)) return;
+    OGSX_GL KrylovState* st = gl(S.st);
+    double red[1];
+    reduce_partials<1>(gl(S.part) + 4 * (size_t)c.G, c.G, red);
+    const double gamma = st->rr;
+    const double alpha_d = gamma / red[0];
+    const T alpha = (T)alpha_d;
+    const GVec<T> v = gvec(kv_of<T>(S));
+    vec_vblocks(c, S.n, [&](int i) {
+        v.x[i] += alpha * v.p[i];
+        v.r[i] -= alpha * v.s[i];
+    });
+    if (c.b == 0 && threadIdx.x == 0) {
+        const int it = st->iter + 1;
+        st->sr_gam = gamma;
+        st->sr_alp = alpha_d;
+        st->sr_it = it;
+        st->done = it >= st->max_iter ? 2 : 0;
+    }
 }
 
 // Element -> CRS slot map, built on the device from the uploaded pattern:

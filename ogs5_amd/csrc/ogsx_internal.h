@@ -96,6 +96,13 @@ struct KrylovState {
                       // 5 reliable-update request, 6 BiCGStab rho breakdown
     int max_iter;
     int pend;         // BiCGStab: converged on s, x updated; done follows
+    // single-reduction CG: the scalars an update kernel hands to the next
+    // iteration (rr_old, alpha, iter take them over in the next SpMV
+    // kernel, so no kernel reads a field that it also writes)
+    double sr_gam;    // (r,r) of the iterate the last update started from
+    double sr_alp;    // alpha of the last update
+    int sr_it;        // iterations completed (>= iter)
+    int sr_pad;
 };
 
 }  // namespace ogsx

@@ -134,8 +134,13 @@ class Server:
             return
         for job in jobs:
             W._prepare_step(job.sc, job.system, job.step, L.DIRICHLET_SYMMETRIC)
-        group.start([j.system for j in jobs], method=L.CG, tol=self.tol,
-                    max_iter=self.max_iter)
+        # a request may carry a tighter iteration limit ($LINEAR_SOLVER
+        # max_iterations of its own deck); one start call per limit
+        limits = {}
+        for j in jobs:
+            limits.setdefault(min(self.max_iter, j.sc.max_iter), []).append(j.system)
+        for max_iter, systems in limits.items():
+            group.start(systems, method=L.CG, tol=self.tol, max_iter=max_iter)
 
     def serve(self, specs: List[W.RequestSpec], keep_x: bool = False,
               log=None) -> dict:
