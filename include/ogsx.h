@@ -295,6 +295,32 @@ int ogsx_solve_status(ogsx_system* s, int* active, ogsx_solve_info* info);
 int ogsx_group_solve(ogsx_group* g, const ogsx_solver_config* cfg,
                      ogsx_solve_info* infos);
 
+/* ---- zero-allocation solve -------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 solve of ogsx_solve (Jacobi-CG or
+ * right-preconditioned BiCGStab, the same kernels and grid, so the same
+ * iterates bit for bit) on a CRS matrix the caller holds in device memory.
+ * row_ptr_dev[n+1], col_dev[nnz] (int32, columns ascending per row as the
+ * CRS pattern builds them) and val_dev[nnz], b_dev[n] are read; x_dev[n] is
+ * the initial guess in and the solution out.  bnorm_inout carries
+ * Linear_EQS::bNorm between solves for the CheckNormRHS test (1.0 for a
+ * first solve, 0 switches the test off; NULL means 1.0); it receives ||b||.
+ * stream: a hipStream_t, or NULL for the null stream; the call returns once
+ * the solve has finished on it.  cfg->method OGSX_CG or OGSX_BICGSTAB;
+ * the FP32_IR fields are ignored.
+ * Limit: n <= 524288 rows (2^19); larger n fails with OGSX_EINVAL.  The
+ * Krylov vectors, partial sums and state live in one fixed device array of
+ * about 38 MB per GPU, held until the process ends; the call makes no
+ * hipMalloc, hipFree or other allocation.  Every call on a GPU uses that
+ * GPU's one array, so calls that overlap on one GPU (other threads, other
+ * streams) race and must be serialised by the caller; each call clears the
+ * state it relies on. */
+int ogsx_solve_crs_noalloc(int64_t n, const int32_t* row_ptr_dev, const int32_t* col_dev,
+                           const double* val_dev, const double* b_dev, double* x_dev,
+                           double* bnorm_inout, const ogsx_solver_config* cfg,
+                           void* stream, ogsx_solve_info* info);
+
 /* ---- measurement ------------------------------------------------------ */
 /* Enables HIP-event timing of every SpMV launch that ogsx_solve issues, of
  * ogsx_assemble and of ogsx_gauss_velocity (on the system's stream) and

@@ -93,6 +93,8 @@ SIGNATURES = {
                                   C.POINTER(C.c_double)]),
     "ogsx_stream_copy": (C.c_int, [_I64, C.c_int, C.POINTER(C.c_double)]),
     "ogsx_synchronize": (C.c_int, [_P]),
+    "ogsx_solve_crs_noalloc": (C.c_int, [_I64, _P, _P, _P, _P, _P, _P,
+                                         C.POINTER(SolverConfig), _P, C.POINTER(SolveInfo)]),
     "ogsx_pattern_footprint": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "ogsx_system_bytes": (C.c_int, [_P, C.POINTER(_I64)]),
     "ogsx_group_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P)]),
@@ -431,3 +433,23 @@ def solve_status(system: "System"):
     check(lib().ogsx_solve_status(system.h, C.byref(a), C.byref(info)), "ogsx_solve_status")
     return bool(a.value), info.as_dict()
 
+
+# Rows the zero-allocation solve takes at most (its fixed device array).
+NOALLOC_MAX_ROWS = 1 << 19
+
+
+def solve_crs_noalloc(n: int, row_ptr_dev: int, col_dev: int, val_dev: int, b_dev: int,
+                      x_dev: int, *, bnorm: float = 1.0, stream: int = 0, **cfg_kw):
+    """ogsx_solve_crs_noalloc on device addresses (ints): the FP64 solve of
+    ogsx_solve on a caller-held CRS matrix, with no allocation.  Returns
+    (info, ||b||); pass ||b|| back as `bnorm` to carry CheckNormRHS's
+    reference between solves."""
+    cfg = solver_config(**cfg_kw)
+    info = SolveInfo()
+    bn = C.c_double(bnorm)
+    check(lib().ogsx_solve_crs_noalloc(int(n), C.c_void_p(row_ptr_dev), C.c_void_p(col_dev),
+                                       C.c_void_p(val_dev), C.c_void_p(b_dev),
+                                       C.c_void_p(x_dev), C.byref(bn), C.byref(cfg),
+                                       C.c_void_p(stream or None), C.byref(info)),
+          "ogsx_solve_crs_noalloc")
+    return info.as_dict(), bn.value

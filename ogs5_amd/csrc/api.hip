@@ -2098,3 +2098,7 @@ int ogsx_synchronize(ogsx_system* s)
 }
 
 }  // extern "C"
+
+// ogsx_solve_crs_noalloc: the zero-allocation FP64 solve over a file-scope
+// scratch array
+#include "noalloc.inc"
