@@ -2110,18 +2110,39 @@ __global__ __launch_bounds__(kBlock) void k_scale_a32(Grp g)
     });
 }
 
-// FP32 SELL / slice-triple values from A32 (tri = 1: the triple layout).
-__global__ __launch_bounds__(kBlock) void k_sell32_from_a32(Grp g, int tri)
+// FP32 SELL-64 / slice-triple values of S A S straight from the FP64 CRS
+// values (tri = 1: the triple layout): one wave per 64-row slice, so the
+// stored entries are written and their source positions read coalesced,
+// and lane l's row is 64 s + l.
+__global__ __launch_bounds__(kBlock) void k_scale_sell32(Grp g, int tri)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
+    const GMat<double> m = gmat(S.m64);
+    const GMat<float> m32 = gmat(S.m32);
     OGSX_GL float* sv = gl(S.sval32);
     const OGSX_GL int* src = gl(tri ? S.tri_src : S.sell_src);
-    const OGSX_GL float* A32 = gl(S.A32);
-    vec_vblocks(c, tri ? S.npad3 : S.npad, [&](int p) {
-        const int k = src[p];
-        sv[p] = k >= 0 ? A32[k] : 0.0f;
-    });
+    const OGSX_GL float* sc = gl(S.v32.d);
+    int sb, se;
+    chunk_of(m32.n_slices, c.G, c.b, sb, se);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int sl = sb + wave; sl < se; sl += kBlock / 64) {
+        int p0, cnt;
+        if (tri) {
+            p0 = m32.tri_ptr[sl];
+            cnt = 3 * m32.tri_cnt[m32.tri_id[sl]];
+        } else {
+            p0 = m32.sell_ptr[sl];
+            cnt = m32.sell_w[sl];
+        }
+        const int row = sl * kSellC + lane;
+        const double si = row < m.n ? (double)sc[row] : 0.0;
+        for (int e = 0; e < cnt; e++) {
+            const int p = p0 + e * kSellC + lane;
+            const int k = src[p];
+            sv[p] = k >= 0 ? (float)(m.val[k] * si * (double)sc[m.col[k]]) : 0.0f;
+        }
+    }
 }
 
 // New recurrence: r = b - (SAS) x; partials (b,b), (r,r), (r,r) for
