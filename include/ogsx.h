@@ -390,6 +390,28 @@ int ogsx_stream_copy(int64_t bytes, int reps, double* gbs);
 /* Synchronises the system's stream. */
 int ogsx_synchronize(ogsx_system* s);
 
+/* ---- persistent launches ---------------------------------------------- */
+/* No counterpart in the reference.  Solves of small systems (and groups of
+ * them) spend their time dispatching the dependent launches of each Krylov
+ * iteration; a persistent launch runs a batch of iterations in one
+ * cooperative launch of co-resident blocks, with a grid barrier between
+ * phases, over the same per-system chunks, so results are bit-identical to
+ * the separate launches.  mode: 0 off (default: on MI355X the grid barrier,
+ * 12-15 us across the 8 XCDs, costs more than the ~4 us kernel boundary it
+ * replaces); 1 auto (running systems of at most 2^19 rows in all, one
+ * slice-based or plain CRS SpMV format, no profiling); 2 whenever the
+ * launch shape allows.  Library-wide setting. */
+int ogsx_set_persistent(int mode);
+/* Current mode and whether the device supports cooperative launches. */
+int ogsx_get_persistent(int* mode, int* supported);
+/* The barrier's waits are bounded by the host: a batch that does not finish
+ * within its deadline (20 s) sets a host flag that releases every wait, the
+ * launch ends normally and the call fails with OGSX_EHIP.  This diagnostic
+ * exercises that path: it launches two blocks that wait for a third (a
+ * barrier that cannot complete) with the flag already set; *aborted = 1
+ * when the blocks gave up and the launch ended. */
+int ogsx_persistent_giveup_probe(int* aborted);
+
 #ifdef __cplusplus
 }
 #endif

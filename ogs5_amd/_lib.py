@@ -93,6 +93,9 @@ SIGNATURES = {
                                   C.POINTER(C.c_double)]),
     "ogsx_stream_copy": (C.c_int, [_I64, C.c_int, C.POINTER(C.c_double)]),
     "ogsx_synchronize": (C.c_int, [_P]),
+    "ogsx_set_persistent": (C.c_int, [C.c_int]),
+    "ogsx_get_persistent": (C.c_int, [_P, _P]),
+    "ogsx_persistent_giveup_probe": (C.c_int, [_P]),
     "ogsx_solve_crs_noalloc": (C.c_int, [_I64, _P, _P, _P, _P, _P, _P,
                                          C.POINTER(SolverConfig), _P, C.POINTER(SolveInfo)]),
     "ogsx_pattern_footprint": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
@@ -453,3 +456,22 @@ def solve_crs_noalloc(n: int, row_ptr_dev: int, col_dev: int, val_dev: int, b_de
                                        C.c_void_p(stream or None), C.byref(info)),
           "ogsx_solve_crs_noalloc")
     return info.as_dict(), bn.value
+
+
+def set_persistent(mode: int):
+    """Persistent Krylov launches: 0 off, 1 auto (small systems), 2 always."""
+    check(lib().ogsx_set_persistent(int(mode)), "ogsx_set_persistent")
+
+
+def get_persistent():
+    """(mode, device supports cooperative launches)."""
+    m, sup = C.c_int(), C.c_int()
+    check(lib().ogsx_get_persistent(C.byref(m), C.byref(sup)), "ogsx_get_persistent")
+    return m.value, bool(sup.value)
+
+
+def persistent_giveup_probe() -> int:
+    """Runs the barrier give-up diagnostic; 1 when the blocks gave up."""
+    a = C.c_int(0)
+    check(lib().ogsx_persistent_giveup_probe(C.byref(a)), "ogsx_persistent_giveup_probe")
+    return a.value

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -14,6 +15,7 @@
 #include <mutex>
 #include <string>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -2088,6 +2090,53 @@ int ogsx_stream_copy(int64_t bytes, int reps, double* gbs)
     (void)hipFree(b);
     HIP_TRY(hipGetLastError());
     return OGSX_OK;
+}
+
+int ogsx_set_persistent(int mode)
+{
+    if (mode < 0 || mode > 2) return fail(OGSX_EINVAL, "persistent mode must be 0, 1 or 2");
+    g_persist_mode.store(mode);
+    return OGSX_OK;
+}
+
+int ogsx_get_persistent(int* mode, int* supported)
+{
+    if (mode) *mode = g_persist_mode.load();
+    if (supported) *supported = coop_supported() ? 1 : 0;
+    return OGSX_OK;
+}
+
+int ogsx_persistent_giveup_probe(int* aborted)
+{
+    if (!aborted) return fail(OGSX_EINVAL, "null argument");
+    if (!coop_supported()) return fail(OGSX_EHIP, "cooperative launches are not supported");
+    GridSync* gs = nullptr;
+    int *ah = nullptr, *ad = nullptr;
+    hipStream_t st = nullptr;
+    int rc = OGSX_OK;
+    if (hipMalloc((void**)&gs, sizeof(GridSync)) != hipSuccess ||
+        hipHostMalloc((void**)&ah, sizeof(int), hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&ad, ah, 0) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        rc = fail(OGSX_EHIP, "set-up of the probe failed");
+    } else {
+        *ah = 1;  // the host has already asked the waits to give up
+        k_zero_sync<<<1, 64, 0, st>>>(gs);
+        void* args[] = {&gs, &ad};
+        hipError_t e = hipLaunchCooperativeKernel((const void*)k_sync_giveup_probe, dim3(2),
+                                                  dim3(kBlock), args, 0, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        GridSync h{};
+        if (e == hipSuccess) e = hipMemcpy(&h, gs, sizeof(h), hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            rc = fail(OGSX_EHIP, std::string("probe: ") + hipGetErrorString(e));
+        else
+            *aborted = h.aborted;
+    }
+    if (st) (void)hipStreamDestroy(st);
+    if (ah) (void)hipHostFree(ah);
+    if (gs) (void)hipFree(gs);
+    return rc;
 }
 
 int ogsx_synchronize(ogsx_system* s)

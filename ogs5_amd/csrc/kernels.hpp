@@ -1306,13 +1306,17 @@ struct Ctx {
     int G, b;  // the block is chunk b of the system's G
 };
 
-__device__ __forceinline__ Ctx ctx_of(const Grp& g)
+// The system and chunk of virtual block vb (a launched block, or one of the
+// blocks a persistent launch works through).
+__device__ __forceinline__ Ctx ctx_of(const Grp& g, int vb)
 {
-    if (!g.blk2sys) return Ctx{g.sys, g.sys->G, (int)blockIdx.x};
-    const int si = g.blk2sys[blockIdx.x];
+    if (!g.blk2sys) return Ctx{g.sys, g.sys->G, vb};
+    const int si = g.blk2sys[vb];
     const SysDev* S = g.sys + si;
-    return Ctx{S, S->G, (int)blockIdx.x - g.blk0[si]};
+    return Ctx{S, S->G, vb - g.blk0[si]};
 }
+
+__device__ __forceinline__ Ctx ctx_of(const Grp& g) { return ctx_of(g, (int)blockIdx.x); }
 
 __device__ __forceinline__ const SysDev* sys_of_block(const Grp& g)
 {
@@ -1464,9 +1468,9 @@ __global__ void k_cg_init_final(Grp g, int want)
 
 // K1: s = A p; partial (p,s)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
+__device__ __forceinline__ void cg_spmv_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     if (gl(S.st)->done) return;
     const GMat<VT> m = gmat(mv_of<VT>(S));
@@ -1482,6 +1486,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
     }
 }
 
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
+{
+    cg_spmv_body<FMT, VT, T>(g, (int)blockIdx.x);
+}
+
 // K2: alpha = rr/(p,s); x += alpha p; r -= alpha s; partials (r,r), (r/d, r).
 // VNT: the vector streams bypass the caches (non-temporal loads and stores),
 // chosen by the host when the matrix of the launch cannot stay in the
@@ -1489,9 +1499,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Grp g)
 // can (128^3 FP64: the solve's SpMV 100.6 -> 87.5 us, the job 549 -> 519
 // ms).  Where matrix and vectors fit, cached streams are faster.
 template <typename T, bool VNT>
-__global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
+__device__ __forceinline__ void cg_update_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
@@ -1566,14 +1576,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
     }
 }
 
+template <typename T, bool VNT>
+__global__ __launch_bounds__(kBlock) void k_cg_update(Grp g)
+{
+    cg_update_body<T, VNT>(g, (int)blockIdx.x);
+}
+
 // K3: error test; beta = (z,r)/rr_old; p = r/d + beta p.  With reliable
 // updates (upd_thresh > 0) the recurrence stops with done = 5 once ||r|| <
 // upd_thresh, leaving p for the resumed iteration.  Every block takes the
 // same branch (same reduced values), so a stop leaves p untouched everywhere.
 template <typename T, bool VNT>
-__global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
+__device__ __forceinline__ void cg_direction_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
@@ -1634,6 +1650,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
         st->iter = it;
         if (it >= st->max_iter) st->done = 2;
     }
+}
+
+template <typename T, bool VNT>
+__global__ __launch_bounds__(kBlock) void k_cg_direction(Grp g)
+{
+    cg_direction_body<T, VNT>(g, (int)blockIdx.x);
 }
 
 // Reliable update: partials (r,r) and (r/d, r) of the replaced residual...
@@ -1744,9 +1766,9 @@ __global__ void k_bi_init_final(Grp g, int want)
 // A convergence on s in B4 (pend) is turned into done here, one kernel
 // later, so that no block of B4 can see it before doing its own update.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
+__device__ __forceinline__ void bi_dir_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
@@ -1799,11 +1821,17 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
     if (c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_dir(Grp g)
+{
+    bi_dir_body<T>(g, (int)blockIdx.x);
+}
+
 // B2: v = A p_h; partial (r0,v); counts the iteration
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(Grp g)
+__device__ __forceinline__ void bi_spmv_v_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
@@ -1821,11 +1849,17 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv_v(Grp g)
     if (c.b == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_v(Grp g)
+{
+    bi_spmv_v_body<FMT, VT, T>(g, (int)blockIdx.x);
+}
+
 // B3: alpha = rho1/(r0,v); s = r - alpha v; s_h = s/d; partial (s,s)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
+__device__ __forceinline__ void bi_s_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
@@ -1852,12 +1886,18 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_s(Grp g)
+{
+    bi_s_body<T>(g, (int)blockIdx.x);
+}
+
 // B4: test on s (then x += alpha p_h and stop: pend, done in B1); else
 // t = A s_h; partials (t,t), (t,s)
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(Grp g)
+__device__ __forceinline__ void bi_spmv_t_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
@@ -1887,12 +1927,18 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv_t(Grp g)
     }
 }
 
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv_t(Grp g)
+{
+    bi_spmv_t_body<FMT, VT, T>(g, (int)blockIdx.x);
+}
+
 // B5: omega = (t,s)/(t,t); x += alpha p_h + omega s_h; r = s - omega t;
 // partials (r,r), (r0,r)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
+__device__ __forceinline__ void bi_update_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
@@ -1915,6 +1961,137 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
         store_partials<2>(acc, gl(S.part), c.G, b);
     }
     if (c.b == 0 && threadIdx.x == 0) st->omega = omega_d;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_update(Grp g)
+{
+    bi_update_body<T>(g, (int)blockIdx.x);
+}
+
+// ---- Persistent launches: a batch of Krylov iterations in one launch
+//
+// For systems too small to fill the GPU the cost of an iteration is the
+// dispatch of its dependent launches, not their work.  A persistent launch
+// keeps a grid of co-resident blocks (cooperative launch) that works
+// through the same virtual blocks -- the same per-system chunks, partial-sum
+// layout and G -- that separate launches would use, so every result is
+// bit-identical, with a grid barrier between two phases instead of a kernel
+// boundary.  Measured on MI355X (200 x 200 quads, FP64 CG): 57 us per
+// iteration against 13 us with three separate launches; the barrier's
+// agent-scope release/acquire across the 8 XCDs costs 12-15 us, a dependent
+// kernel dispatch about 4 us, so the path is off unless asked for.
+
+// Barrier state of one persistent launch (device memory, zeroed before it).
+struct GridSync {
+    unsigned count;  // blocks arrived at the current barrier
+    unsigned gen;    // barriers completed
+    int aborted;     // some block gave up waiting (the host asked it to)
+    int pad;
+};
+
+// How often a waiting block looks at the host's abort flag (polls).
+constexpr unsigned kSyncAbortPolls = 4096;
+
+// Waits until `target` blocks arrived (target = gridDim.x).  The wait is
+// bounded by the host: every kSyncAbortPolls polls a waiting block reads
+// the host's abort flag and, when it is set, leaves (returns false); a
+// launch whose blocks leave ends normally and the host reports the
+// failure.  Agent-scope release/acquire fences make the phase's writes of
+// every block (on every XCD) visible to the next phase.
+__device__ bool grid_sync(GridSync* gs, const volatile int* host_abort, unsigned target)
+{
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ok = 1;
+        const unsigned g0 = __hip_atomic_load(&gs->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();  // the block's writes (all its waves passed the barrier above)
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived + 1 == target) {
+            __hip_atomic_store(&gs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned polls = 0;
+            while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++polls % kSyncAbortPolls == 0 &&
+                    (*host_abort ||
+                     __hip_atomic_load(&gs->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    __hip_atomic_store(&gs->aborted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    return ok;
+}
+
+// Phases of the recurrences a persistent launch runs.
+constexpr int kPersistCg = 0;    // K1 SpMV, K2 update, K3 direction
+constexpr int kPersistCgSr = 1;  // single-reduction CG: K1 SpMV, K2 update
+constexpr int kPersistBi = 2;    // BiCGStab: B1 .. B5
+
+// Recurrence state `done` of a single-system launch, read after a barrier.
+__device__ __forceinline__ int stopped(const Grp& g)
+{
+    return __hip_atomic_load(&g.sys->st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// `iters` iterations of the recurrence over the virtual blocks of gm (SpMV
+// phases, nbm blocks) and gv (vector phases, nbv blocks).  A single system
+// (gm.blk2sys == nullptr) leaves the loop once its recurrence stopped.  The
+// flag is read right after the barrier that follows the only phase writing
+// it (CG: K3; single-reduction CG: K2; BiCGStab: B1), so no block can be
+// writing it while another reads it: every block leaves at the same point.
+template <int FMT, typename VT, typename T, int KM>
+__global__ __launch_bounds__(kBlock) void k_krylov_persistent(Grp gm, int nbm, Grp gv, int nbv,
+                                                              int iters, GridSync* gs,
+                                                              const int* host_abort)
+{
+    const unsigned all = gridDim.x;
+    const volatile int* ha = host_abort;
+    const bool single = gm.blk2sys == nullptr;
+#define OGSX_PHASE(NB, CALL)                                                  \
+    for (int vb = (int)blockIdx.x; vb < (NB); vb += (int)gridDim.x) {        \
+        CALL;                                                                 \
+        __syncthreads();                                                      \
+    }                                                                         \
+    if (!grid_sync(gs, ha, all)) return;
+    for (int it = 0; it < iters; it++) {
+        if constexpr (KM == kPersistCg) {
+            OGSX_PHASE(nbm, (cg_spmv_body<FMT, VT, T>(gm, vb)))
+            OGSX_PHASE(nbv, (cg_update_body<T, false>(gv, vb)))
+            OGSX_PHASE(nbv, (cg_direction_body<T, false>(gv, vb)))
+            if (single && stopped(gm)) break;
+        } else if constexpr (KM == kPersistCgSr) {
+            OGSX_PHASE(nbm, (cg1_spmv_body<FMT, VT, T>(gm, 0, vb)))
+            OGSX_PHASE(nbv, (cg1_update_body<T>(gv, vb)))
+            if (single && stopped(gm)) break;
+        } else {
+            OGSX_PHASE(nbv, (bi_dir_body<T>(gv, vb)))
+            if (single && stopped(gm)) break;
+            OGSX_PHASE(nbm, (bi_spmv_v_body<FMT, VT, T>(gm, vb)))
+            OGSX_PHASE(nbv, (bi_s_body<T>(gv, vb)))
+            OGSX_PHASE(nbm, (bi_spmv_t_body<FMT, VT, T>(gm, vb)))
+            OGSX_PHASE(nbv, (bi_update_body<T>(gv, vb)))
+        }
+    }
+#undef OGSX_PHASE
+}
+
+// Diagnostic of the barrier's give-up path: every block waits for one more
+// block than the grid holds, a barrier that cannot complete; with the host's
+// abort flag already set, each block leaves after kSyncAbortPolls polls and
+// the launch ends normally.
+__global__ void k_sync_giveup_probe(GridSync* gs, const int* host_abort)
+{
+    const volatile int* ha = host_abort;
+    (void)grid_sync(gs, ha, gridDim.x + 1);
 }
 
 // ---- FP32-inside-FP64 iterative refinement helpers
@@ -2184,9 +2361,9 @@ __global__ __launch_bounds__(kBlock) void k_cg1_init(Grp g, int want)
 // the last update left (sr_*).  resume = 1: the first SpMV after a reliable
 // update, run for the systems whose refinement step chose to resume.
 template <int FMT, typename VT, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg1_spmv(Grp g, int resume)
+__device__ __forceinline__ void cg1_spmv_body(const Grp& g, int resume, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (resume ? skip_action(S, 2) : st->done != 0) return;
@@ -2207,13 +2384,19 @@ __global__ __launch_bounds__(kBlock) void k_cg1_spmv(Grp g, int resume)
     }
 }
 
+template <int FMT, typename VT, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_spmv(Grp g, int resume)
+{
+    cg1_spmv_body<FMT, VT, T>(g, resume, (int)blockIdx.x);
+}
+
 // K2: stopping test on the current r, then the update.  With reliable
 // updates (upd_thresh > 0) the recurrence stops with done = 5 once ||r|| <
 // upd_thresh, before updating, leaving p and s for the resumption.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg1_update(Grp g)
+__device__ __forceinline__ void cg1_update_body(const Grp& g, int vb)
 {
-    const Ctx c = ctx_of(g);
+    const Ctx c = ctx_of(g, vb);
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done) return;
@@ -2255,7 +2438,16 @@ __global__ __launch_bounds__(kBlock) void k_cg1_update(Grp g)
         }
         return;
     }
+    // the limit is tested here, where every block reads the same count, so
+    // either all blocks update or none does
     const int it = st->iter;
+    if (it >= st->max_iter) {
+        if (c.b == 0 && threadIdx.x == 0) {
+            st->error = error;
+            st->done = 2;
+        }
+        return;
+    }
     const bool start = it == 0;
     double alpha_d, beta_d = 0.0;
     if (start) {
@@ -2301,8 +2493,13 @@ __global__ __launch_bounds__(kBlock) void k_cg1_update(Grp g)
         st->sr_alp = alpha_d;
         st->sr_it = it + 1;
         st->error = error;
-        if (it + 1 >= st->max_iter) st->done = 2;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_update(Grp g)
+{
+    cg1_update_body<T>(g, (int)blockIdx.x);
 }
 
 // Resumption after a reliable update (r replaced by the scaled true
