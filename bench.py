@@ -113,23 +113,29 @@ def gpu_leg(sc, precision, steps, warmup):
     )
 
 
-def error_vs_fp64_ref(sc, system, legs, tol_ref=1e-13):
+def error_vs_fp64_ref(sc, system, legs, tol_ref=1e-13, method=None, dirichlet=None,
+                      with_ref=False):
     """Relative L2 error of every step's solution of each leg against the
-    same job solved in FP64 to a tight residual (the device FP64 CG, whose
-    solutions match the oracle within 1e-10 at every size the oracle can
-    run).  Errors include what earlier steps pass on through u_old."""
+    same job solved in FP64 to a tight residual (the device FP64 solver of
+    the job -- Jacobi-CG by default -- whose solutions match the oracle within
+    1e-10 at every size the oracle can run).  Errors include what earlier
+    steps pass on through u_old.  with_ref: returns (errors, the reference's
+    step solutions)."""
     import ogs5_amd as ox
     from ogs5_amd import workloads as W
-    ref = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=ox.FP64,
+    method = ox.CG if method is None else method
+    dirichlet = ox.DIRICHLET_SYMMETRIC if dirichlet is None else dirichlet
+    ref = W.run_gpu(sc, method=method, dirichlet=dirichlet, precision=ox.FP64,
                     system=system, ic_on_device=True, fetch_x=False, keep_steps=True,
                     tol=tol_ref)
     assert all(i["converged"] == 1 for i in ref["info"]), ref["info"]
-    out = dict(reference=f"FP64 Jacobi-CG job to {tol_ref:g} relative residual on the device",
+    name = "Jacobi-CG" if method == ox.CG else "Jacobi-BiCGStab"
+    out = dict(reference=f"FP64 {name} job to {tol_ref:g} relative residual on the device",
                reference_residuals=[i["error"] for i in ref["info"]])
     for name, xs in legs.items():
         out[name] = [float(np.linalg.norm(x - r) / np.linalg.norm(r))
                      for x, r in zip(xs, ref["steps"])]
-    return out
+    return (out, ref["steps"]) if with_ref else out
 
 
 def cpu_baseline(sc, iters_per_solve):
@@ -367,6 +373,7 @@ def heat_mode(size, steps, cpu=True):
     sc = W.tet_heat(size, n_steps=steps)
     warm = W.Scenario(**{**sc.__dict__, "steps": sc.steps[:1]})
     res = {}
+    steps_x = {}
     for name, prec in (("fp64", ox.FP64), ("fp32_ir", ox.FP32_IR)):
         system = W.make_system(sc, prec)
         pat, s = system
@@ -382,11 +389,12 @@ def heat_mode(size, steps, cpu=True):
         info = out["info"]
         assert all(i["converged"] == 1 and i["error"] <= 1e-10 for i in info), info
         s.profile(True)
-        W.run_gpu(sc, **kw)
+        steps_x[name] = W.run_gpu(sc, keep_steps=True, **kw)["steps"]
         kind = ox._lib.PROF_SPMV64 if prec == ox.FP64 else ox._lib.PROF_SPMV32
         ms, launches, bpl = s.profile_read(kind)
         s.profile(False)
         k_ms, k_bytes = s.bench_spmv(prec == ox.FP32_IR, 50)
+        nnz, n = s.nnz, s.n
         res[name] = dict(seconds=dt, ms_per_step=1e3 * dt / steps,
                          mdof_solves_per_s=steps * sc.n / dt / 1e6,
                          iterations=[i["iterations"] for i in info],
@@ -398,9 +406,19 @@ def heat_mode(size, steps, cpu=True):
                          spmv_format=s.spmv_format(prec == ox.FP32_IR))
         _log(f"heat {name}: {dt:.2f} s, {steps * sc.n / dt / 1e6:.1f} MDOF-solves/s, "
              f"iterations {res[name]['iterations'][:4]}...")
+        if name == "fp64":
+            # every step of both legs against a tight FP64 BiCGStab job (untimed)
+            errors, ref_x = error_vs_fp64_ref(sc, system, {}, tol_ref=1e-12, method=ox.BICGSTAB,
+                                              dirichlet=ox.DIRICHLET_ROW, with_ref=True)
         s.close()
+    for name, xs in steps_x.items():
+        errors[name] = [float(np.linalg.norm(x - r) / np.linalg.norm(r)) for x, r in zip(xs, ref_x)]
     best = res["fp32_ir"] if res["fp32_ir"]["seconds"] < res["fp64"]["seconds"] else res["fp64"]
     sp = best["spmv"]
+    # algorithmic bytes of one SpMV: every nonzero value read once, x read
+    # once, y written once
+    width = 4 if best is res["fp32_ir"] else 8
+    alg = width * nnz + 2 * width * n
     cpu_base = None
     if cpu:
         from oracle import ffi
@@ -444,11 +462,13 @@ def heat_mode(size, steps, cpu=True):
         "config": {"workload": f"HEAT_TRANSPORT, {size}^3 x 6 tet4, Crank-Nicolson, {steps} steps, "
                                f"Jacobi-BiCGStab, row-only Dirichlet", "dof": sc.n,
                    "elements": sc.mesh.n_elem},
-        "roofline": dict(bound="hbm", achieved=sp["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=sp["gbs"] / HBM_PEAK_GBS, traffic=None,
+        "roofline": dict(bound="hbm", achieved=alg / sp["avg_ms"] / 1e6, peak=HBM_PEAK_GBS,
+                         unit="GB/s", frac=alg / sp["avg_ms"] / 1e6 / HBM_PEAK_GBS, traffic=None,
                          kernel="k_cg_spmv<float>" if best is res["fp32_ir"] else "k_cg_spmv<double>",
-                         avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"],
+                         avg_launch_ms=sp["avg_ms"], bytes_per_launch=alg,
+                         stored_bytes_per_launch=sp["bytes_per_launch"],
                          format=best["spmv_format"]),
+        "error_vs_fp64_ref": errors,
         "cpu_baseline": cpu_base,
         "runs": res,
     }))
@@ -631,15 +651,24 @@ def main():
     best_name = "fp32_ir" if best is ir else "fp64"
     sp = best["spmv"]
     peak = HBM_PEAK_GBS
-    roof = dict(bound="hbm", achieved=sp["gbs"], peak=peak, unit="GB/s",
-                frac=(sp["gbs"] / peak) if sp["gbs"] else None, traffic=None,
+    # algorithmic bytes of one SpMV: every nonzero value read once, x read
+    # once, y written once (no index, padding or stored zero is counted; the
+    # tuned format's own stored bytes ride along as stored_bytes_per_launch)
+    width = 4 if best is ir else 8
+    alg = width * _nnz(sc) + 2 * width * sc.n
+    achieved = alg / sp["avg_ms"] / 1e6
+    in_ms = best["spmv_in_solve"]["avg_ms"]
+    roof = dict(bound="hbm", achieved=achieved, peak=peak, unit="GB/s",
+                frac=achieved / peak, traffic=None,
                 kernel=("k_cg_spmv<float>" if best is ir else "k_cg_spmv<double>"),
-                avg_launch_ms=sp["avg_ms"], bytes_per_launch=sp["bytes_per_launch"],
+                avg_launch_ms=sp["avg_ms"], bytes_per_launch=alg,
+                stored_bytes_per_launch=sp["bytes_per_launch"],
+                frac_stored=sp["gbs"] / peak if sp["gbs"] else None,
                 format=best["spmv_format"],
                 # the same kernel inside the solve (events around every launch
                 # of one untimed job): p is rewritten between launches
-                frac_in_solve=best["spmv_in_solve"]["frac"],
-                avg_launch_ms_in_solve=best["spmv_in_solve"]["avg_ms"])
+                frac_in_solve=alg / in_ms / 1e6 / peak if in_ms else None,
+                avg_launch_ms_in_solve=in_ms)
     # HBM bytes per dispatch of this kernel and format, from the FETCH_SIZE /
     # WRITE_SIZE passes over `bench.py --mode spmv` (profiles/spmv_hex128_traffic.md)
     tf = os.path.join(ROOT, "profiles", "traffic.json")

@@ -295,6 +295,29 @@ int ogsx_solve_status(ogsx_system* s, int* active, ogsx_solve_info* info);
 int ogsx_group_solve(ogsx_group* g, const ogsx_solver_config* cfg,
                      ogsx_solve_info* infos);
 
+/* One time step of `count` members in batched launches, then their solves
+ * (as ogsx_group_start).  Per member: u_old <- x when shift is set (the
+ * copy-back of the previous step, ogsx_copy_vector), Initialize, the
+ * element loop of ogsx_assemble (dt, theta, steady), its last source list
+ * (ogsx_add_rhs) when sources is set, and its last Dirichlet list
+ * (ogsx_set_dirichlet) with mode dirichlet (-1: none) -- the calls of
+ * CRFProcess::Execute for that step (rf_pcs.cpp:4695-4853), each phase one
+ * launch over all listed members (one per element class for the element
+ * passes), with each member's A and b bit-identical to the per-system
+ * calls.  The lists are those the members' last ogsx_add_rhs /
+ * ogsx_set_dirichlet uploaded.  Members need the two-pass assembly (rows of
+ * at most 32 entries).  No counterpart in the reference. */
+typedef struct {
+    double dt;
+    double theta;
+    int steady;
+    int dirichlet;  /* -1, OGSX_DIRICHLET_ROW or OGSX_DIRICHLET_SYMMETRIC */
+    int sources;    /* 1: add the stored source list                      */
+    int shift;      /* 1: u_old <- x first                                */
+} ogsx_step_params;
+int ogsx_group_step(ogsx_group* g, ogsx_system* const* systems, int count,
+                    const ogsx_step_params* params, const ogsx_solver_config* cfg);
+
 /* ---- zero-allocation solve -------------------------------------------- */
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

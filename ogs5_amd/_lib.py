@@ -105,6 +105,7 @@ SIGNATURES = {
     "ogsx_group_add": (C.c_int, [_P, _P]),
     "ogsx_group_remove": (C.c_int, [_P, _P]),
     "ogsx_group_start": (C.c_int, [_P, C.POINTER(_P), C.c_int, C.POINTER(SolverConfig)]),
+    "ogsx_group_step": (C.c_int, [_P, C.POINTER(_P), C.c_int, _P, C.POINTER(SolverConfig)]),
     "ogsx_group_advance": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     "ogsx_solve_status": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(SolveInfo)]),
     "ogsx_group_solve": (C.c_int, [_P, C.POINTER(SolverConfig), C.POINTER(SolveInfo)]),
@@ -378,6 +379,12 @@ class System:
             pass
 
 
+class StepParams(C.Structure):
+    """ogsx_step_params: one member's time step in ogsx_group_step."""
+    _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("steady", C.c_int),
+                ("dirichlet", C.c_int), ("sources", C.c_int), ("shift", C.c_int)]
+
+
 class Group:
     """Systems solved together (ogsx_group_*): one launch per Krylov phase
     over all running members; each member's results are those of
@@ -403,6 +410,22 @@ class Group:
         arr = (C.c_void_p * len(systems))(*[s.h.value for s in systems])
         c = solver_config(**cfg)
         check(lib().ogsx_group_start(self.h, arr, len(systems), C.byref(c)), "ogsx_group_start")
+
+    def step(self, systems, params, **cfg):
+        """ogsx_group_step: one time step of each listed system in batched
+        launches (params: one dict per system with dt, theta, steady,
+        dirichlet (-1 for none), sources, shift), then their solves."""
+        if not systems:
+            return
+        arr = (C.c_void_p * len(systems))(*[s.h.value for s in systems])
+        ps = (StepParams * len(systems))()
+        for k, p in enumerate(params):
+            ps[k] = StepParams(float(p["dt"]), float(p["theta"]), int(bool(p.get("steady", 0))),
+                               int(p.get("dirichlet", -1)), int(bool(p.get("sources", 0))),
+                               int(bool(p.get("shift", 0))))
+        c = solver_config(**cfg)
+        check(lib().ogsx_group_step(self.h, arr, len(systems), C.cast(ps, C.c_void_p),
+                                    C.byref(c)), "ogsx_group_step")
 
     def advance(self, max_iterations: int = 1 << 30) -> int:
         n = C.c_int()

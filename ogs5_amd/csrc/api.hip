@@ -184,7 +184,7 @@ struct ogsx_system {
     // allocated on the first assembly that uses them
     int *inc_ptr = nullptr, *inc_ele = nullptr, *asm_pos = nullptr;
     uint8_t* asm_loc = nullptr;
-    double *asm_eb = nullptr, *asm_rb = nullptr;
+    double* asm_eb = nullptr;  // records [E*npe][npe + 1 or npe + 2]
     // values
     double* A = nullptr;
     float* A32 = nullptr;
@@ -641,7 +641,7 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
     // right-hand sides
     if (!h.inc_ptr.empty() && h.row_max <= kAsmRowMax) {
         const int64_t np = h.npe;
-        t += b(n + 1, 4) + 2 * b(E * np, 4) + b(E * np * np, 1) + b(E * np * np, 8) + b(E * np * 2, 8);
+        t += b(n + 1, 4) + 2 * b(E * np, 4) + b(E * np * np, 1) + b(E * np * (np + 2), 8);
     }
     return t;
 }
@@ -1207,9 +1207,19 @@ void launch_assemble_colours(ogsx_system* s, const AsmArgs& a)
         if (h.n_elem > 0)
             k_assemble<ET, PROC, ISO, true><<<(int)((h.n_elem + 63) / 64), 64, 0, s->stream>>>(
                 a, 0, (int)h.n_elem);
-        const int grav = PROC == OGSX_LIQUID_FLOW && a.rho_g != 0.0;
-        k_assemble_rows<ET><<<(s->n + kAsmRowBlock - 1) / kAsmRowBlock, kAsmRowBlock, 0, s->stream>>>(
-            a, s->inc_ptr, s->asm_loc, s->ptr, s->n, grav);
+        const bool grav = a.rs == ETraits<ET>::NPE + 2;
+        const int lanes = asm_row_lanes(h.row_max);
+        const int blocks = (int)(((int64_t)s->n * lanes + kAsmRowThreads - 1) / kAsmRowThreads);
+        if (blocks == 0) return;
+#define ROWS_(GR, L)                                                                         \
+    k_assemble_rows<ET, GR, L><<<blocks, kAsmRowThreads, 0, s->stream>>>(a, s->inc_ptr, s->asm_loc, \
+                                                                        s->ptr, s->n)
+        if (lanes == 16) {
+            if (grav) ROWS_(true, 16); else ROWS_(false, 16);
+        } else {
+            if (grav) ROWS_(true, 32); else ROWS_(false, 32);
+        }
+#undef ROWS_
         return;
     }
     for (int c = 0; c < h.n_colors; c++) {
@@ -1231,8 +1241,7 @@ int alloc_two_pass(ogsx_system* s)
     OK_TRY(dalloc(s, &s->inc_ele, h.inc_ele.size()));
     OK_TRY(dalloc(s, &s->asm_pos, h.inc_ele.size()));
     OK_TRY(dalloc(s, &s->asm_loc, (size_t)cnt));
-    OK_TRY(dalloc(s, &s->asm_rb, (size_t)(h.n_elem * npe * 2)));
-    OK_TRY(dalloc(s, &s->asm_eb, (size_t)cnt));
+    OK_TRY(dalloc(s, &s->asm_eb, (size_t)(h.n_elem * npe * (npe + 2))));
     HIP_TRY(hipMemcpyAsync(s->inc_ptr, h.inc_ptr.data(), 4 * h.inc_ptr.size(), hipMemcpyHostToDevice,
                            s->stream));
     HIP_TRY(hipMemcpyAsync(s->inc_ele, h.inc_ele.data(), 4 * h.inc_ele.size(), hipMemcpyHostToDevice,
@@ -1361,13 +1370,19 @@ int ogsx_system_initialize(ogsx_system* s)
     return OGSX_OK;
 }
 
-int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
+}  // extern "C"
+
+namespace {
+// The element pass computes gravity terms (LIQUID_FLOW with rho g != 0;
+// GROUNDWATER_FLOW's head form has rho_g = 0).
+bool asm_args_gravity(const ogsx_system* s)
 {
-    if (!s) return fail(OGSX_EINVAL, "null system");
-    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
-    if (!s->elements_set) return fail(OGSX_ESTATE, "ogsx_system_set_elements not called");
-    if (!(dt > 0.0)) return fail(OGSX_EINVAL, "dt must be positive");
-    s->a_symmetric = s->tensor_sym && !(s->process == OGSX_HEAT_TRANSPORT && s->use_vel);
+    return s->process == OGSX_LIQUID_FLOW && s->rho_g != 0.0;
+}
+
+// Arguments of the assembly kernels for system s (dt, theta, steady).
+AsmArgs asm_args(const ogsx_system* s, double dt, double theta, int steady)
+{
     AsmArgs a;
     std::memset(&a, 0, sizeof(a));
     a.conn = s->conn;
@@ -1397,9 +1412,23 @@ int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
     a.theta = theta;
     if (s->asm_eb) {  // two-pass assembly (buffers from ogsx_system_set_elements)
         a.eb = s->asm_eb;
-        a.rb = s->asm_rb;
         a.pos = s->asm_pos;
+        a.rs = s->pat->npe + (asm_args_gravity(s) ? 2 : 1);
     }
+    return a;
+}
+}  // namespace
+
+extern "C" {
+
+int ogsx_assemble(ogsx_system* s, double dt, double theta, int steady)
+{
+    if (!s) return fail(OGSX_EINVAL, "null system");
+    if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
+    if (!s->elements_set) return fail(OGSX_ESTATE, "ogsx_system_set_elements not called");
+    if (!(dt > 0.0)) return fail(OGSX_EINVAL, "dt must be positive");
+    s->a_symmetric = s->tensor_sym && !(s->process == OGSX_HEAT_TRANSPORT && s->use_vel);
+    const AsmArgs a = asm_args(s, dt, theta, steady);
     int pk = prof_begin(s, OGSX_PROF_ASSEMBLY);
     switch (s->pat->etype) {
         case OGSX_LINE2: launch_assemble_et<OGSX_LINE2>(s, a); break;
@@ -1606,6 +1635,22 @@ struct ogsx_group {
     int precision = OGSX_FP64;
     hipStream_t stream = nullptr;
     MemberSet* set = nullptr;
+    // batched step preparation: the members' StepArgs (pinned staging and
+    // device copy), the event after its upload, and the event after the
+    // preparation that the members' streams wait on
+    StepArgs* tab_host = nullptr;
+    StepArgs* tab_dev = nullptr;
+    size_t tab_cap = 0;
+    hipEvent_t tab_copied = nullptr;
+    hipEvent_t prep_done = nullptr;
+    ~ogsx_group()
+    {
+        if (tab_copied) (void)hipEventSynchronize(tab_copied);
+        if (tab_host) (void)hipHostFree(tab_host);
+        if (tab_dev) (void)hipFree(tab_dev);
+        if (tab_copied) (void)hipEventDestroy(tab_copied);
+        if (prep_done) (void)hipEventDestroy(prep_done);
+    }
 };
 
 namespace {
@@ -1814,6 +1859,249 @@ int ogsx_group_start(ogsx_group* g, ogsx_system* const* systems, int count,
         HIP_TRY(hipStreamWaitEvent(g->stream, s->ready, 0));
     }
     return set_start(m, which);
+}
+
+}  // extern "C"
+
+namespace {
+// Block maps of one phase of the batched step preparation: per listed
+// member (compact index) its block count; returns the total block count.
+int upload_step_map(MemberSet* m, const std::vector<int64_t>& blocks, const int** blk2mem,
+                    const int** blk0, int* total)
+{
+    std::vector<int> host;
+    int64_t t = 0;
+    for (int64_t b : blocks) t += b;
+    if (t >= INT_MAX) return fail(OGSX_EINVAL, "batched step: too many blocks");
+    host.reserve((size_t)t + blocks.size() + 1);
+    std::vector<int> b0(blocks.size() + 1, 0);
+    for (size_t i = 0; i < blocks.size(); i++) {
+        b0[i] = (int)host.size();
+        for (int64_t k = 0; k < blocks[i]; k++) host.push_back((int)i);
+    }
+    b0[blocks.size()] = (int)host.size();
+    const size_t nb = host.size();
+    host.insert(host.end(), b0.begin(), b0.end());
+    OK_TRY(arena_reserve(m, m->tmp_arena, host.size()));
+    int* dev = nullptr;
+    OK_TRY(upload_ints(m, m->tmp_arena, host.data(), host.size(), &dev));
+    *blk2mem = dev;
+    *blk0 = dev + nb;
+    *total = (int)nb;
+    return OGSX_OK;
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+template <int ET, int PROC, bool ISO>
+void launch_step_assemble(hipStream_t st, const StepMap& sm, int blocks)
+{
+    k_step_assemble<ET, PROC, ISO><<<blocks, 64, 0, st>>>(sm);
+}
+
+template <int ET>
+void launch_step_rows(hipStream_t st, const StepMap& sm, int blocks, bool grav, int lanes)
+{
+    if (lanes == 16) {
+        if (grav)
+            k_step_rows<ET, true, 16><<<blocks, kAsmRowThreads, 0, st>>>(sm);
+        else
+            k_step_rows<ET, false, 16><<<blocks, kAsmRowThreads, 0, st>>>(sm);
+    } else {
+        if (grav)
+            k_step_rows<ET, true, 32><<<blocks, kAsmRowThreads, 0, st>>>(sm);
+        else
+            k_step_rows<ET, false, 32><<<blocks, kAsmRowThreads, 0, st>>>(sm);
+    }
+}
+
+template <int ET>
+void launch_step_assemble_et(hipStream_t st, const StepMap& sm, int blocks, bool heat, bool iso)
+{
+    if (heat) {
+        if (iso)
+            launch_step_assemble<ET, OGSX_HEAT_TRANSPORT, true>(st, sm, blocks);
+        else
+            launch_step_assemble<ET, OGSX_HEAT_TRANSPORT, false>(st, sm, blocks);
+    } else {
+        if (iso)
+            launch_step_assemble<ET, OGSX_LIQUID_FLOW, true>(st, sm, blocks);
+        else
+            launch_step_assemble<ET, OGSX_LIQUID_FLOW, false>(st, sm, blocks);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int ogsx_group_step(ogsx_group* g, ogsx_system* const* systems, int count,
+                    const ogsx_step_params* params, const ogsx_solver_config* cfg)
+{
+    if (!g || !cfg || count < 0 || (count > 0 && (!systems || !params)))
+        return fail(OGSX_EINVAL, "bad argument");
+    if (count == 0) return OGSX_OK;
+    OK_TRY(check_config(cfg));
+    MemberSet* m = g->set;
+    for (int k = 0; k < count; k++) {
+        ogsx_system* s = systems[k];
+        const ogsx_step_params& p = params[k];
+        if (member_index(g, s) < 0) return fail(OGSX_EINVAL, "system is not a member of this group");
+        if (s->sc.active) return fail(OGSX_ESTATE, "system has a solve in progress");
+        if (!s->elements_set) return fail(OGSX_ESTATE, "ogsx_system_set_elements not called");
+        if (!(p.dt > 0.0)) return fail(OGSX_EINVAL, "dt must be positive");
+        if (p.dirichlet != -1 && p.dirichlet != OGSX_DIRICHLET_ROW &&
+            p.dirichlet != OGSX_DIRICHLET_SYMMETRIC)
+            return fail(OGSX_EINVAL, "unknown Dirichlet mode");
+        if (!s->asm_eb || !use_two_pass(s))
+            return fail(OGSX_EINVAL, "batched steps need the two-pass assembly (rows of at most 32 "
+                                     "entries)");
+        for (int j = 0; j < k; j++)
+            if (systems[j] == s) return fail(OGSX_EINVAL, "system listed twice");
+    }
+    // the members' queued work (uploads, the last copy-back) comes first
+    for (int k = 0; k < count; k++) {
+        ogsx_system* s = systems[k];
+        if (!s->ready) HIP_TRY(hipEventCreateWithFlags(&s->ready, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(s->ready, s->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, s->ready, 0));
+    }
+    // the members' arguments, through pinned staging (the previous call's
+    // copy must have left it)
+    if ((size_t)count > g->tab_cap) {
+        if (g->tab_copied) HIP_TRY(hipEventSynchronize(g->tab_copied));
+        if (g->tab_host) (void)hipHostFree(g->tab_host);
+        if (g->tab_dev) (void)hipFree(g->tab_dev);
+        g->tab_host = nullptr;
+        g->tab_dev = nullptr;
+        g->tab_cap = 0;
+        const size_t cap = std::max<size_t>(count, 64);
+        if (hipHostMalloc((void**)&g->tab_host, cap * sizeof(StepArgs)) != hipSuccess ||
+            hipMalloc((void**)&g->tab_dev, cap * sizeof(StepArgs)) != hipSuccess)
+            return fail(OGSX_ENOMEM, "allocation of the step table failed");
+        g->tab_cap = cap;
+    }
+    if (!g->tab_copied) HIP_TRY(hipEventCreateWithFlags(&g->tab_copied, hipEventDisableTiming));
+    if (!g->prep_done) HIP_TRY(hipEventCreateWithFlags(&g->prep_done, hipEventDisableTiming));
+    HIP_TRY(hipEventSynchronize(g->tab_copied));
+    for (int k = 0; k < count; k++) {
+        ogsx_system* s = systems[k];
+        const ogsx_step_params& p = params[k];
+        StepArgs& A = g->tab_host[k];
+        std::memset(&A, 0, sizeof(A));
+        A.a = asm_args(s, p.dt, p.theta, p.steady);
+        A.inc_ptr = s->inc_ptr;
+        A.loc = s->asm_loc;
+        A.ptr = s->ptr;
+        A.col = s->col;
+        A.diagpos = s->diagpos;
+        A.st_node = s->st_node_d;
+        A.st_vptr = s->st_vptr_d;
+        A.st_val = s->st_val_d;
+        A.bc_idx = s->bc_idx_d;
+        A.bc_val = s->bc_val_d;
+        A.bcflag = s->bcflag;
+        A.bcg = s->bcg;
+        A.uold = s->uold;
+        A.x = s->v64.x;
+        A.n = s->n;
+        A.nnz = (int)s->nnz;
+        A.ne = (int)s->pat->n_elem;
+        A.st_nu = p.sources ? s->st_nu : 0;
+        A.bc_m = p.dirichlet != -1 ? s->bc_m : 0;
+        A.shift = p.shift != 0;
+    }
+    HIP_TRY(hipMemcpyAsync(g->tab_dev, g->tab_host, sizeof(StepArgs) * count,
+                           hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipEventRecord(g->tab_copied, g->stream));
+    hipStream_t st = g->stream;
+    auto phase = [&](auto blocks_of, auto launch) -> int {
+        std::vector<int64_t> nb(count);
+        int64_t any = 0;
+        for (int k = 0; k < count; k++) any += (nb[k] = blocks_of(k));
+        if (any == 0) return OGSX_OK;
+        StepMap sm{g->tab_dev, nullptr, nullptr};
+        int total = 0;
+        OK_TRY(upload_step_map(m, nb, &sm.blk2mem, &sm.blk0, &total));
+        launch(sm, total);
+        return OGSX_OK;
+    };
+    // u_old <- x, A = 0, b = 0 (ogsx_copy_vector, Linear_EQS::Initialize)
+    OK_TRY(phase([&](int k) { return ceil_div(std::max<int64_t>(systems[k]->nnz, systems[k]->n), kStepFill); },
+                 [&](const StepMap& sm, int total) { k_step_init<<<total, kBlock, 0, st>>>(sm); }));
+    // the element pass, one launch per (element type, process, tensor) class
+    for (int et : {OGSX_LINE2, OGSX_QUAD4, OGSX_HEX8, OGSX_TET4})
+        for (int heat = 0; heat < 2; heat++)
+            for (int iso = 0; iso < 2; iso++) {
+                auto in = [&](int k) {
+                    const ogsx_system* s = systems[k];
+                    return s->pat->etype == et && (s->process == OGSX_HEAT_TRANSPORT) == (heat != 0) &&
+                           (s->cond_stride == 1) == (iso != 0);
+                };
+                OK_TRY(phase([&](int k) { return in(k) ? ceil_div(systems[k]->pat->n_elem, 64) : 0; },
+                             [&](const StepMap& sm, int total) {
+                                 switch (et) {
+                                     case OGSX_LINE2: launch_step_assemble_et<OGSX_LINE2>(st, sm, total, heat, iso); break;
+                                     case OGSX_QUAD4: launch_step_assemble_et<OGSX_QUAD4>(st, sm, total, heat, iso); break;
+                                     case OGSX_HEX8: launch_step_assemble_et<OGSX_HEX8>(st, sm, total, heat, iso); break;
+                                     default: launch_step_assemble_et<OGSX_TET4>(st, sm, total, heat, iso); break;
+                                 }
+                             }));
+            }
+    // the row pass, per (element type, gravity, lanes per row)
+    for (int et : {OGSX_LINE2, OGSX_QUAD4, OGSX_HEX8, OGSX_TET4})
+        for (int grav = 0; grav < 2; grav++)
+            for (int lanes : {16, 32}) {
+                auto in = [&](int k) {
+                    const ogsx_system* s = systems[k];
+                    const bool gr = asm_args_gravity(s);
+                    return s->pat->etype == et && gr == (grav != 0) &&
+                           asm_row_lanes(s->pat->row_max) == lanes;
+                };
+                OK_TRY(phase(
+                    [&](int k) { return in(k) ? ceil_div((int64_t)systems[k]->n * lanes, kAsmRowThreads) : 0; },
+                    [&](const StepMap& sm, int total) {
+                        switch (et) {
+                            case OGSX_LINE2: launch_step_rows<OGSX_LINE2>(st, sm, total, grav, lanes); break;
+                            case OGSX_QUAD4: launch_step_rows<OGSX_QUAD4>(st, sm, total, grav, lanes); break;
+                            case OGSX_HEX8: launch_step_rows<OGSX_HEX8>(st, sm, total, grav, lanes); break;
+                            default: launch_step_rows<OGSX_TET4>(st, sm, total, grav, lanes); break;
+                        }
+                    }));
+            }
+    // IncorporateSourceTerms with the stored lists
+    OK_TRY(phase([&](int k) { return params[k].sources ? ceil_div(systems[k]->st_nu, kBlock) : 0; },
+                 [&](const StepMap& sm, int total) { k_step_sources<<<total, kBlock, 0, st>>>(sm); }));
+    // Dirichlet with the stored lists: row-only members, then symmetric ones
+    auto bc_blocks = [&](int mode) {
+        return [&, mode](int k) { return params[k].dirichlet == mode ? ceil_div(systems[k]->bc_m, kBlock) : 0; };
+    };
+    OK_TRY(phase(bc_blocks(OGSX_DIRICHLET_ROW),
+                 [&](const StepMap& sm, int total) { k_step_bc_row<<<total, kBlock, 0, st>>>(sm); }));
+    OK_TRY(phase(bc_blocks(OGSX_DIRICHLET_SYMMETRIC), [&](const StepMap& sm, int total) {
+        k_step_bc_mark<<<total, kBlock, 0, st>>>(sm, (int8_t)1);
+    }));
+    OK_TRY(phase(
+        [&](int k) {
+            return params[k].dirichlet == OGSX_DIRICHLET_SYMMETRIC && systems[k]->bc_m > 0
+                       ? std::min<int64_t>(ceil_div(systems[k]->n, kBlock / 32), 2048)
+                       : 0;
+        },
+        [&](const StepMap& sm, int total) { k_step_bc_sym<<<total, kBlock, 0, st>>>(sm); }));
+    OK_TRY(phase(bc_blocks(OGSX_DIRICHLET_SYMMETRIC), [&](const StepMap& sm, int total) {
+        k_step_bc_mark<<<total, kBlock, 0, st>>>(sm, (int8_t)0);
+    }));
+    HIP_TRY(hipGetLastError());
+    // the members' own work (format tuning, value copies, the solve start)
+    // follows the preparation
+    HIP_TRY(hipEventRecord(g->prep_done, g->stream));
+    for (int k = 0; k < count; k++) {
+        ogsx_system* s = systems[k];
+        HIP_TRY(hipStreamWaitEvent(s->stream, g->prep_done, 0));
+        s->a_version++;
+        s->a_symmetric = s->tensor_sym && !(s->process == OGSX_HEAT_TRANSPORT && s->use_vel) &&
+                         !(params[k].dirichlet == OGSX_DIRICHLET_ROW && s->bc_m > 0);
+    }
+    return ogsx_group_start(g, systems, count, cfg);
 }
 
 int ogsx_group_advance(ogsx_group* g, int max_iterations, int* n_active)

@@ -19,6 +19,8 @@
 
 #include <float.h>
 
+#include <type_traits>
+
 #include "ogsx_internal.h"
 
 namespace ogsx {
@@ -720,13 +722,15 @@ struct AsmArgs {
     const double* u_old;
     double* A;
     double* b;
-    double* eb;         // two-pass assembly: element rows [E*npe][npe] and
-    double* rb;         // right-hand sides [E*npe][2] (rhs, gravity), stored
-    const int32_t* pos; // at the (element, node) pair's incidence position pos
+    double* eb;         // two-pass assembly: one record per (element, node)
+                        // pair at its incidence position pos: the element's
+                        // row, its right-hand side and, with gravity, its
+                        // gravity term (rs = npe + 1 or npe + 2 doubles)
+    const int32_t* pos;
     int cond_stride;
     int ngp;
     int steady;
-    int pad;
+    int rs;             // record stride (doubles)
     double tuf, mu, geo_area, adv_coef, rho_g, dt, theta;
 };
 
@@ -821,12 +825,9 @@ __device__ __forceinline__ double elem_gradients(const double* X, const double* 
 // reference's k,l loop add exact zeros and are skipped, and K is formed on
 // its lower triangle and mirrored.
 template <int ET, int PROC, bool ISO, bool STORE>
-__global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
+__device__ __forceinline__ void assemble_elem(const AsmArgs& a, int e)
 {
     constexpr int NPE = ETraits<ET>::NPE, D = ETraits<ET>::D, NW = ETraits<ET>::NW;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    const int e = STORE ? off + t : a.color_elems[off + t];
     int nd[NPE];
     double X[NPE], Y[NPE], Z[NPE];
 #pragma unroll
@@ -867,11 +868,24 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
 #pragma unroll
     for (int i = 0; i < NPE; i++) G[i] = 0.0;
 
+    // Linear tets and lines have the same local gradients at every Gauss
+    // point, so the Jacobian, its determinant and the global gradients are
+    // the same at every point: computed once, with the values the per-point
+    // computation gives.
+    constexpr bool kConstGrad = ET == OGSX_TET4 || ET == OGSX_LINE2;
+    double dshp0[D * NPE], det0 = 0.0;
+    if constexpr (kConstGrad) det0 = elem_gradients<ET>(X, Y, Z, a.tabdN, dshp0);
     for (int gp = 0; gp < a.ngp; gp++) {
         const double* Ng = a.tabN + gp * NPE;
-        const double* dNg = a.tabdN + gp * D * NPE;
         double dshp[D * NPE];
-        const double det = elem_gradients<ET>(X, Y, Z, dNg, dshp);
+        double det;
+        if constexpr (kConstGrad) {
+            det = det0;
+#pragma unroll
+            for (int i = 0; i < D * NPE; i++) dshp[i] = dshp0[i];
+        } else {
+            det = elem_gradients<ET>(X, Y, Z, a.tabdN + gp * D * NPE, dshp);
+        }
         double fkt = fabs(det);
         if (D != 3) fkt *= a.geo_area;
         fkt = fkt * a.tabW[gp * 3 + 0];
@@ -969,16 +983,17 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
         }
     }
     if (STORE) {
-        // two-pass assembly: each element row goes to its node's incidence
-        // list (node-major), where k_assemble_rows reads it in sequence
+        // two-pass assembly: each element row goes, with its right-hand
+        // side (and gravity term), to its node's incidence list (node-major),
+        // where k_assemble_rows reads it in sequence
 #pragma unroll
         for (int i = 0; i < NPE; i++) {
             const int64_t q = a.pos[(int64_t)e * NPE + i];
-            double* eb = a.eb + q * NPE;
+            double* rec = a.eb + q * a.rs;
 #pragma unroll
-            for (int j = 0; j < NPE; j++) eb[j] = lhs[i * NPE + j];
-            a.rb[2 * q] = rhs[i];
-            a.rb[2 * q + 1] = do_grav ? G[i] : 0.0;
+            for (int j = 0; j < NPE; j++) rec[j] = lhs[i * NPE + j];
+            rec[NPE] = rhs[i];
+            if (do_grav) rec[NPE + 1] = G[i];
         }
         return;
     }
@@ -993,53 +1008,105 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
     }
 }
 
-// Second pass of the two-pass assembly: one thread per row r adds the rows
-// of the element matrices that belong to node r, element by element in
-// ascending element order (the order of the reference's element loop,
-// FEM/rf_pcs.cpp:5741-5762), and within an element by local node; no two
-// threads touch one entry, so no colouring and no atomics.  The element
-// rows sit in node r's incidence order, so a thread reads one contiguous
-// stretch and neighbouring rows read neighbouring stretches.  The row is
-// accumulated in LDS (loc: an entry's position in its row) starting from
-// A's current values, so every entry is summed in the same order as
-// A += contribution element by element.  b gets the element right-hand
-// side, then the gravity term, as the scatter does.
-constexpr int kAsmRowMax = 32;     // longest row the two-pass assembly takes
-constexpr int kAsmRowBlock = 128;  // threads (rows) per block
-template <int ET>
-__global__ __launch_bounds__(kAsmRowBlock) void k_assemble_rows(AsmArgs a, const int32_t* inc_ptr,
-                                                                const uint8_t* loc, const int32_t* ptr,
-                                                                int n, int grav)
+template <int ET, int PROC, bool ISO, bool STORE>
+__global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int e = STORE ? off + t : a.color_elems[off + t];
+    assemble_elem<ET, PROC, ISO, STORE>(a, e);
+}
+
+// Second pass of the two-pass assembly: G lanes per row r, lane c owning
+// entry c of the row.  The rows of the element matrices that belong to node
+// r sit in node r's incidence order (ascending element order, the order of
+// the reference's element loop, FEM/rf_pcs.cpp:5741-5762); the lanes walk
+// them together, record by record, and lane c adds the one value of each
+// record whose column position (loc) is c.  Every entry is therefore summed
+// from A's current value in the same order as A += contribution element by
+// element, and no two lanes touch one entry (no colouring, no atomics).  The
+// row's lanes read each record together (one contiguous stretch per row), so
+// a wave holds a few cache lines at a time and each line is fetched once;
+// a lane issues the loads of 8 records before it adds them, so a row costs a
+// few memory round trips rather than two per record.  Lane 0 adds the
+// element right-hand sides, then the gravity terms, to b[r] as the scatter
+// does.
+constexpr int kAsmRowMax = 32;      // longest row the two-pass assembly takes
+constexpr int kAsmRowThreads = 256; // threads per block of the row pass
+
+// lanes per row for rows of at most row_max entries
+__host__ __device__ constexpr int asm_row_lanes(int row_max) { return row_max <= 16 ? 16 : 32; }
+
+// Column positions of one record (NPE bytes at q NPE) as one word; words
+// past the row's last record read as 0xff bytes, which match no lane.
+template <int NPE>
+using LocWord = typename std::conditional<NPE == 2, uint16_t,
+                typename std::conditional<NPE == 4, uint32_t, uint64_t>::type>::type;
+
+template <int NPE>
+__device__ __forceinline__ int loc_match(uint64_t w, int c)
+{
+    int jm = -1;
+#pragma unroll
+    for (int j = 0; j < NPE; j++) jm = (int)((w >> (8 * j)) & 0xff) == c ? j : jm;
+    return jm;
+}
+
+constexpr int kAsmRowBatch = 8;  // records whose loads a lane issues together
+
+template <int ET, bool GRAV>
+__device__ __forceinline__ void assemble_row(const AsmArgs& a, const int32_t* inc_ptr,
+                                             const uint8_t* loc, const int32_t* ptr, int r, int c)
 {
     constexpr int NPE = ETraits<ET>::NPE;
-    __shared__ double acc[kAsmRowMax * kAsmRowBlock];
-    const int tid = threadIdx.x;
-    const int r = blockIdx.x * kAsmRowBlock + tid;
-    if (r >= n) return;
+    constexpr int RS = NPE + (GRAV ? 2 : 1);
+    constexpr int B = kAsmRowBatch;
+    using W = LocWord<NPE>;
     const OGSX_GL double* eb = gl(a.eb);
-    const OGSX_GL double* rb = gl(a.rb);
-    const OGSX_GL uint8_t* lc = gl(loc);
+    const OGSX_GL W* lw = (const OGSX_GL W*)gl(loc);
     OGSX_GL double* A = gl(a.A);
     const int p0 = ptr[r], len = ptr[r + 1] - p0;
-    for (int c = 0; c < len; c++) acc[c * kAsmRowBlock + tid] = A[p0 + c];
-    double br = gl(a.b)[r];
-    const int q1 = inc_ptr[r + 1];
-    for (int64_t q = inc_ptr[r]; q < q1; q++) {
-        const int64_t base = q * NPE;
-        double v[NPE];
-        int l[NPE];
+    const bool mine = c < len;
+    double acc = mine ? A[p0 + c] : 0.0;
+    double br = c == 0 ? gl(a.b)[r] : 0.0;
+    const int64_t q0 = inc_ptr[r], q1 = inc_ptr[r + 1];
+    for (int64_t q = q0; q < q1; q += B) {
+        // all loads of the batch first, then the sums in record order
+        uint64_t w[B];
 #pragma unroll
-        for (int j = 0; j < NPE; j++) {
-            v[j] = eb[base + j];
-            l[j] = lc[base + j];
+        for (int k = 0; k < B; k++) w[k] = q + k < q1 ? (uint64_t)lw[q + k] : ~0ull;
+        double v[B], rb[B], gb[B];
+        int jm[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            jm[k] = mine ? loc_match<NPE>(w[k], c) : -1;
+            const OGSX_GL double* rec = eb + (q + k) * RS;
+            v[k] = jm[k] >= 0 ? rec[jm[k]] : 0.0;
+            const bool rh = c == 0 && q + k < q1;
+            rb[k] = rh ? rec[NPE] : 0.0;
+            gb[k] = GRAV && rh ? rec[NPE + 1] : 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < NPE; j++) acc[l[j] * kAsmRowBlock + tid] += v[j];
-        br += rb[2 * q];
-        if (grav) br += rb[2 * q + 1];
+        for (int k = 0; k < B; k++) {
+            if (jm[k] >= 0) acc += v[k];
+            if (c == 0 && q + k < q1) {
+                br += rb[k];
+                if (GRAV) br += gb[k];
+            }
+        }
     }
-    for (int c = 0; c < len; c++) A[p0 + c] = acc[c * kAsmRowBlock + tid];
-    gl(a.b)[r] = br;
+    if (mine) A[p0 + c] = acc;
+    if (c == 0) gl(a.b)[r] = br;
+}
+
+template <int ET, bool GRAV, int G>
+__global__ __launch_bounds__(kAsmRowThreads) void k_assemble_rows(AsmArgs a, const int32_t* inc_ptr,
+                                                                  const uint8_t* loc,
+                                                                  const int32_t* ptr, int n)
+{
+    const int r = (int)((blockIdx.x * (int64_t)kAsmRowThreads + threadIdx.x) / G);
+    if (r >= n) return;
+    assemble_row<ET, GRAV>(a, inc_ptr, loc, ptr, r, (int)threadIdx.x % G);
 }
 
 // Two-pass assembly tables, per incidence position q (node-major, inc_ei[q] =
@@ -2048,6 +2115,11 @@ __device__ __forceinline__ int stopped(const Grp& g)
 // flag is read right after the barrier that follows the only phase writing
 // it (CG: K3; single-reduction CG: K2; BiCGStab: B1), so no block can be
 // writing it while another reads it: every block leaves at the same point.
+template <int FMT, typename VT, typename T>
+__device__ __forceinline__ void cg1_spmv_body(const Grp& g, int resume, int vb);
+template <typename T>
+__device__ __forceinline__ void cg1_update_body(const Grp& g, int vb);
+
 template <int FMT, typename VT, typename T, int KM>
 __global__ __launch_bounds__(kBlock) void k_krylov_persistent(Grp gm, int nbm, Grp gv, int nbv,
                                                               int iters, GridSync* gs,
@@ -2622,34 +2694,43 @@ __global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const T* a, T* b)
 }
 
 // Point source terms: one thread per distinct node, values in list order.
-__global__ __launch_bounds__(kBlock) void k_add_rhs(int nu, const int* node, const int* vptr,
-                                                    const double* val, double* b)
+__device__ __forceinline__ void add_rhs_node(int k, const int* node, const int* vptr,
+                                             const double* val, double* b)
 {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= nu) return;
     double acc = b[node[k]];
     for (int j = vptr[k]; j < vptr[k + 1]; j++) acc += val[j];
     b[node[k]] = acc;
 }
 
+__global__ __launch_bounds__(kBlock) void k_add_rhs(int nu, const int* node, const int* vptr,
+                                                    const double* val, double* b)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nu) return;
+    add_rhs_node(k, node, vptr, val, b);
+}
+
 // Dirichlet flags: flag[i] = 1 and g[i] for every listed node
+__device__ __forceinline__ void bc_mark_one(int k, const int* idx, const double* val, int8_t* flag,
+                                            double* g, int8_t on)
+{
+    flag[idx[k]] = on;
+    if (on) g[idx[k]] = val[k];
+}
+
 __global__ __launch_bounds__(kBlock) void k_bc_mark(int m, const int* idx, const double* val,
                                                     int8_t* flag, double* g, int8_t on)
 {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= m) return;
-    flag[idx[k]] = on;
-    if (on) g[idx[k]] = val[k];
+    bc_mark_one(k, idx, val, flag, g, on);
 }
 
 // Row-only elimination (CSparseMatrix::Diagonize CRS branch,
 // matrix_class.cpp:1663-1690,1772): one thread per Dirichlet node.
-__global__ __launch_bounds__(kBlock) void k_bc_row(int m, const int* idx, const double* val,
-                                                   const int* ptr, const int* col,
-                                                   const int* diagpos, double* A, double* b)
+__device__ __forceinline__ void bc_row_one(int k, const int* idx, const double* val, const int* ptr,
+                                           const int* col, const int* diagpos, double* A, double* b)
 {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= m) return;
     const int id = idx[k];
     const int dp = diagpos[id];
     const double vdiag = dp >= 0 ? A[dp] : 0.0;
@@ -2658,19 +2739,27 @@ __global__ __launch_bounds__(kBlock) void k_bc_row(int m, const int* idx, const 
     b[id] = vdiag * val[k];
 }
 
+__global__ __launch_bounds__(kBlock) void k_bc_row(int m, const int* idx, const double* val,
+                                                   const int* ptr, const int* col,
+                                                   const int* diagpos, double* A, double* b)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m) return;
+    bc_row_one(k, idx, val, ptr, col, diagpos, A, b);
+}
+
 // Symmetric elimination: Dirichlet rows as above; every other row lifts
 // b_i -= a_ik g_k and zeroes a_ik for Dirichlet columns k (column order).
 // 32 lanes per row: the row's columns are read in one coalesced pass, and
 // the lifts are applied one Dirichlet column at a time in column order
 // (ballot over the lanes that hold one, then each lane's product in turn).
-__global__ __launch_bounds__(kBlock) void k_bc_sym(int n, const int* ptr, const int* col,
-                                                   const int* diagpos, const int8_t* flag,
-                                                   const double* g, double* A, double* b)
+__device__ __forceinline__ void bc_sym_rows(int n, const int* ptr, const int* col,
+                                            const int* diagpos, const int8_t* flag, const double* g,
+                                            double* A, double* b, int first, int step)
 {
     const int lane = threadIdx.x & 31;
     const int half = (threadIdx.x & 63) >> 5;
-    const int step = gridDim.x * (kBlock / 32);
-    for (int i = blockIdx.x * (kBlock / 32) + (int)(threadIdx.x >> 5); i < n; i += step) {
+    for (int i = first; i < n; i += step) {
         const int p0 = ptr[i], p1 = ptr[i + 1];
         if (flag[i]) {
             const int dp = diagpos[i];
@@ -2701,6 +2790,140 @@ __global__ __launch_bounds__(kBlock) void k_bc_sym(int n, const int* ptr, const 
             if (lane == 0 && touched) b[i] = bi;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bc_sym(int n, const int* ptr, const int* col,
+                                                   const int* diagpos, const int8_t* flag,
+                                                   const double* g, double* A, double* b)
+{
+    bc_sym_rows(n, ptr, col, diagpos, flag, g, A, b,
+                blockIdx.x * (kBlock / 32) + (int)(threadIdx.x >> 5), gridDim.x * (kBlock / 32));
+}
+
+// ------------------------------------------------------------------------
+// Batched time-step preparation over the members of a group: one launch per
+// phase (u_old <- x and A = b = 0; element pass; row pass; sources;
+// Dirichlet) over every listed member, each member's blocks doing exactly
+// what the per-system kernels do for it, so its A and b are bit-identical.
+// ------------------------------------------------------------------------
+struct StepArgs {
+    AsmArgs a;
+    const int32_t* inc_ptr;  // two-pass incidence and row positions
+    const uint8_t* loc;
+    const int32_t* ptr;      // CRS
+    const int32_t* col;
+    const int32_t* diagpos;
+    const int32_t* st_node;  // stored source list (grouped by node)
+    const int32_t* st_vptr;
+    const double* st_val;
+    const int32_t* bc_idx;   // stored Dirichlet list
+    const double* bc_val;
+    int8_t* bcflag;
+    double* bcg;
+    double* uold;
+    const double* x;
+    int n, nnz, ne, st_nu, bc_m, shift;
+};
+
+// Launch geometry of one phase: block b works for member blk2mem[b] as its
+// block b - blk0[m] of blk0[m + 1] - blk0[m].
+struct StepMap {
+    const StepArgs* tab;
+    const int* blk2mem;
+    const int* blk0;
+};
+
+struct StepCtx {
+    const StepArgs* S;
+    int lb, nb;
+};
+
+__device__ __forceinline__ StepCtx step_ctx(const StepMap& m)
+{
+    const int mi = m.blk2mem[blockIdx.x];
+    const int b0 = m.blk0[mi];
+    return StepCtx{m.tab + mi, (int)blockIdx.x - b0, m.blk0[mi + 1] - b0};
+}
+
+constexpr int kStepFill = kBlock * 4;  // entries per block of the fill phase
+
+// u_old <- x (with shift), A = 0, b = 0.
+__global__ __launch_bounds__(kBlock) void k_step_init(StepMap m)
+{
+    const StepCtx c = step_ctx(m);
+    const StepArgs& S = *c.S;
+    OGSX_GL double* A = gl(S.a.A);
+    OGSX_GL double* b = gl(S.a.b);
+    OGSX_GL double* uo = gl(S.uold);
+    const OGSX_GL double* x = gl(S.x);
+    const int shift = S.shift;
+    for (int i = c.lb * kStepFill + (int)threadIdx.x; i < (c.lb + 1) * kStepFill; i += kBlock) {
+        if (i < S.nnz) A[i] = 0.0;
+        if (i < S.n) {
+            b[i] = 0.0;
+            if (shift) uo[i] = x[i];
+        }
+    }
+}
+
+template <int ET, int PROC, bool ISO>
+__global__ __launch_bounds__(64) void k_step_assemble(StepMap m)
+{
+    const StepCtx c = step_ctx(m);
+    const int t = c.lb * 64 + (int)threadIdx.x;
+    if (t >= c.S->ne) return;
+    const AsmArgs a = c.S->a;
+    assemble_elem<ET, PROC, ISO, true>(a, t);
+}
+
+template <int ET, bool GRAV, int G>
+__global__ __launch_bounds__(kAsmRowThreads) void k_step_rows(StepMap m)
+{
+    const StepCtx c = step_ctx(m);
+    const StepArgs& S = *c.S;
+    const int r = (int)((c.lb * (int64_t)kAsmRowThreads + threadIdx.x) / G);
+    if (r >= S.n) return;
+    const AsmArgs a = S.a;
+    assemble_row<ET, GRAV>(a, S.inc_ptr, S.loc, S.ptr, r, (int)threadIdx.x % G);
+}
+
+// b[node] += the node's source values in list order (as k_add_rhs)
+__global__ __launch_bounds__(kBlock) void k_step_sources(StepMap m)
+{
+    const StepCtx c = step_ctx(m);
+    const StepArgs& S = *c.S;
+    const int k = c.lb * kBlock + (int)threadIdx.x;
+    if (k >= S.st_nu) return;
+    add_rhs_node(k, S.st_node, S.st_vptr, S.st_val, S.a.b);
+}
+
+// row-only Dirichlet elimination (as k_bc_row)
+__global__ __launch_bounds__(kBlock) void k_step_bc_row(StepMap m)
+{
+    const StepCtx c = step_ctx(m);
+    const StepArgs& S = *c.S;
+    const int k = c.lb * kBlock + (int)threadIdx.x;
+    if (k >= S.bc_m) return;
+    bc_row_one(k, S.bc_idx, S.bc_val, S.ptr, S.col, S.diagpos, S.a.A, S.a.b);
+}
+
+// Dirichlet flags on / off (as k_bc_mark)
+__global__ __launch_bounds__(kBlock) void k_step_bc_mark(StepMap m, int8_t on)
+{
+    const StepCtx c = step_ctx(m);
+    const StepArgs& S = *c.S;
+    const int k = c.lb * kBlock + (int)threadIdx.x;
+    if (k >= S.bc_m) return;
+    bc_mark_one(k, S.bc_idx, S.bc_val, S.bcflag, S.bcg, on);
+}
+
+// symmetric elimination (as k_bc_sym) over the member's rows with its blocks
+__global__ __launch_bounds__(kBlock) void k_step_bc_sym(StepMap m)
+{
+    const StepCtx c = step_ctx(m);
+    const StepArgs& S = *c.S;
+    bc_sym_rows(S.n, S.ptr, S.col, S.diagpos, S.bcflag, S.bcg, S.a.A, S.a.b,
+                c.lb * (kBlock / 32) + (int)(threadIdx.x >> 5), c.nb * (kBlock / 32));
 }
 
 }  // namespace ogsx

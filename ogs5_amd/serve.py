@@ -13,9 +13,12 @@ more thread, off the GPU loop.
 
 Each step of a request is OGS-5's time step on its own system: Initialize
 -> assembly -> source terms -> Dirichlet -> solve -> copy-back
-(CRFProcess::Execute, FEM/rf_pcs.cpp:4695-4853); the group solve returns the
-same iterates as a solo solve, so every result is bit-identical to the
-request solved alone.
+(CRFProcess::Execute, FEM/rf_pcs.cpp:4695-4853).  The steps after a
+request's first are prepared for all the requests that start one together
+in one batched set of launches (ogsx_group_step), with each request's A and
+b bit-identical to its own calls; the group solve returns the same iterates
+as a solo solve, so every result is bit-identical to the request solved
+alone.
 
 Latency of a request = completion - arrival (queueing, host setup and all
 its steps).
@@ -128,19 +131,32 @@ class Server:
         self.max_iter = max_iter
 
     def _begin_steps(self, group, jobs: List[Job]):
-        """Assembles each job's next step on its own stream, then starts all
-        their solves with one group call (one set of launches for all)."""
+        """Prepares each job's next step and starts its solve: a first step
+        (which uploads the request's source and Dirichlet lists) or a restart
+        from the initial condition on the job's own stream, every other step
+        of all the jobs in one batched preparation (Group.step, u_old <- x
+        included); one group call starts the solves."""
         if not jobs:
             return
-        for job in jobs:
-            W._prepare_step(job.sc, job.system, job.step, L.DIRICHLET_SYMMETRIC)
         # a request may carry a tighter iteration limit ($LINEAR_SOLVER
         # max_iterations of its own deck); one start call per limit
         limits = {}
         for j in jobs:
-            limits.setdefault(min(self.max_iter, j.sc.max_iter), []).append(j.system)
-        for max_iter, systems in limits.items():
-            group.start(systems, method=L.CG, tol=self.tol, max_iter=max_iter)
+            limits.setdefault(min(self.max_iter, j.sc.max_iter), []).append(j)
+        for max_iter, grp in limits.items():
+            many = [j for j in grp if W.batchable(j.sc, j.step)]
+            one = [j for j in grp if not W.batchable(j.sc, j.step)]
+            for j in one:
+                if j.step > 0:
+                    j.system.copy_vector(L.VEC_UOLD, L.VEC_X)
+                W._prepare_step(j.sc, j.system, j.step, L.DIRICHLET_SYMMETRIC)
+            if one:
+                group.start([j.system for j in one], method=L.CG, tol=self.tol,
+                            max_iter=max_iter)
+            if many:
+                group.step([j.system for j in many],
+                           [W.step_params(j.sc, j.step, L.DIRICHLET_SYMMETRIC) for j in many],
+                           method=L.CG, tol=self.tol, max_iter=max_iter)
 
     def serve(self, specs: List[W.RequestSpec], keep_x: bool = False,
               log=None) -> dict:
@@ -210,7 +226,6 @@ class Server:
                         continue
                     job.infos.append(info)
                     s = job.system
-                    s.copy_vector(L.VEC_UOLD, L.VEC_X)
                     job.step += 1
                     if job.step < job.spec.steps:
                         stepping.append(job)

@@ -382,15 +382,64 @@ def _prepare_step(sc: Scenario, s, k: int, dirichlet: int, ic_on_device: bool = 
     s.set_dirichlet(sc.bc_idx, sc.bc_val, dirichlet)
 
 
+def step_params(sc: Scenario, k: int, dirichlet: int) -> dict:
+    """Group.step parameters of step k > 0 of a scenario (not a restart from
+    the initial condition): u_old <- x, then the steps of _prepare_step with
+    the lists the scenario's first step uploaded."""
+    st = sc.steps[k]
+    return dict(dt=st.dt, theta=st.theta, steady=st.steady, dirichlet=dirichlet,
+                sources=bool(sc.st_idx.size), shift=True)
+
+
+def batchable(sc: Scenario, k: int) -> bool:
+    """Step k can go through Group.step: not the first step (which uploads
+    the source and Dirichlet lists) and not a restart from the initial
+    condition."""
+    return k > 0 and not sc.steps[k].reset_to_ic
+
+
+def _start_steps(g, scs, systems, ids, step, dirichlet, batched, **cfg):
+    """Prepares and starts step step[i] of every listed instance: through one
+    Group.step for the batchable ones (batched), else on each instance's
+    own stream (its copy-back u_old <- x first, for k > 0), then one group
+    start per distinct (tol, max_iter)."""
+    tol = cfg.pop("tol")
+    by = {}
+    for i in ids:
+        sc = scs[i]
+        by.setdefault((sc.tol if tol is None else tol, sc.max_iter), []).append(i)
+    for (t, mi), grp in by.items():
+        many = [i for i in grp if batched and batchable(scs[i], step[i])]
+        one = [i for i in grp if i not in many]
+        for i in one:
+            s = systems[i][1]
+            if step[i] > 0:
+                s.copy_vector(L.VEC_UOLD, L.VEC_X)
+            _prepare_step(scs[i], s, step[i], dirichlet)
+        if one:
+            g.start([systems[i][1] for i in one], tol=t, max_iter=mi, **cfg)
+        if many:
+            g.step([systems[i][1] for i in many],
+                   [step_params(scs[i], step[i], dirichlet) for i in many], tol=t, max_iter=mi,
+                   **cfg)
+
+
 def run_group(scs: List[Scenario], *, method: int = L.CG,
               dirichlet: int = L.DIRICHLET_SYMMETRIC, precision: int = L.FP64,
               inner_tol: Optional[float] = None, tol: Optional[float] = None,
               ir_mode: Optional[int] = None, use_graphs: bool = True, keep_steps=False,
-              stagger: int = 0, systems=None) -> dict:
-    """Runs the scenarios' steps in lock-step through one ogsx group: per
-    step every instance is assembled on its own stream, then all solves run
-    together.  stagger > 0 starts the instances one at a time with that many
-    group iterations in between (solves join while others are running)."""
+              stagger: int = 0, systems=None, continuous: bool = False,
+              batch_iters: int = 64, batched_steps: bool = False) -> dict:
+    """Runs the scenarios' steps through one ogsx group.  Lock-step (default):
+    per step every instance is assembled on its own stream, then all solves
+    run together.  stagger > 0 starts the instances one at a time with that
+    many group iterations in between (solves join while others are
+    running).  continuous: an instance assembles and starts its next step as
+    soon as its solve finished, while the others keep running (each
+    instance's arithmetic is its own, so results are those of lock-step).
+    batched_steps: steps after the first (except restarts from the initial
+    condition) are prepared for all starting instances at once
+    (Group.step), with the same A, b and results."""
     if systems is None:
         systems = [make_system(sc, precision) for sc in scs]
     g = L.Group(precision, use_graphs)
@@ -400,16 +449,19 @@ def run_group(scs: List[Scenario], *, method: int = L.CG,
         s.set_vector(L.VEC_X, sc.u0)
     infos = [[] for _ in scs]
     xs = [[] for _ in scs]
+    cfg = dict(method=method, tol=tol, inner_tol=inner_tol, ir_mode=ir_mode)
+    if continuous:
+        return _run_group_continuous(scs, systems, g, infos, xs, dirichlet, keep_steps,
+                                     batch_iters, batched_steps, cfg)
     for k in range(max(len(sc.steps) for sc in scs)):
         live = [i for i, sc in enumerate(scs) if k < len(sc.steps)]
-        for i in live:
-            _prepare_step(scs[i], systems[i][1], k, dirichlet)
-        for i in live:
-            sc = scs[i]
-            g.start([systems[i][1]], method=method, tol=sc.tol if tol is None else tol,
-                    max_iter=sc.max_iter, inner_tol=inner_tol, ir_mode=ir_mode)
-            if stagger:
+        step = [k] * len(scs)
+        if stagger:
+            for i in live:
+                _start_steps(g, scs, systems, [i], step, dirichlet, batched_steps, **cfg)
                 g.advance(stagger)
+        else:
+            _start_steps(g, scs, systems, live, step, dirichlet, batched_steps, **cfg)
         while g.advance() > 0:
             pass
         for i in live:
@@ -417,9 +469,43 @@ def run_group(scs: List[Scenario], *, method: int = L.CG,
             active, info = L.solve_status(s)
             assert not active
             infos[i].append(info)
-            s.copy_vector(L.VEC_UOLD, L.VEC_X)
             if keep_steps:
                 xs[i].append(s.get_vector(L.VEC_X))
+    out = dict(x=[s.get_vector(L.VEC_X) for _, s in systems], info=infos, systems=systems,
+               group=g)
+    if keep_steps:
+        out["steps"] = xs
+    return out
+
+
+def _run_group_continuous(scs, systems, g, infos, xs, dirichlet, keep_steps, batch_iters,
+                          batched_steps, cfg):
+    step = [0] * len(scs)
+
+    def start(ids):
+        _start_steps(g, scs, systems, ids, step, dirichlet, batched_steps, **cfg)
+
+    live = [i for i, sc in enumerate(scs) if sc.steps]
+    start(live)
+    while live:
+        g.advance(batch_iters)
+        nxt, still = [], []
+        for i in live:
+            s = systems[i][1]
+            active, info = L.solve_status(s)
+            if active:
+                still.append(i)
+                continue
+            infos[i].append(info)
+            if keep_steps:
+                xs[i].append(s.get_vector(L.VEC_X))
+            step[i] += 1
+            if step[i] < len(scs[i].steps):
+                nxt.append(i)
+                still.append(i)
+        if nxt:
+            start(nxt)
+        live = still
     out = dict(x=[s.get_vector(L.VEC_X) for _, s in systems], info=infos, systems=systems,
                group=g)
     if keep_steps:

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# OGSX_ORACLE_LIB: another build of oracle.c (the sanitizer runs use one)
+LIB_PATH = os.environ.get("OGSX_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_DUMP = os.path.join(HERE, "_ref", "ogs5_ref_dump")
 
 LINE2, QUAD4, HEX8, TET4 = 1, 2, 3, 5

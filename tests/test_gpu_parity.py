@@ -21,12 +21,14 @@ ENTRY_TOL = 1e-12
 SOL_TOL = 1e-10
 # Relative L2 error bounds of the full-size solutions (FP64 and FP32-inside-
 # FP64 refinement, both stopped at a 1e-10 relative residual) against a
-# 1e-13 FP64 solve of the same system: twice the measured errors (128^3
-# hexes, sigma_lnK = 2: FP64 6.7e-9, IR 9.4e-9; 96^3 x 6 tets: FP64
-# 1.3e-11, IR 5.4e-14).  The hex system's condition number turns a 1e-10
-# residual into an error near 1e-8.
+# 1e-13 FP64 solve of the same system.  Measured: 128^3 hexes, sigma_lnK = 2:
+# FP64 6.7e-9, IR 9.4e-9 (the condition number turns a 1e-10 residual into an
+# error near 1e-8); 96^3 x 6 tets: FP64 1.3e-11 and 3.2e-11 (two builds whose
+# element kernels contract different products to FMA: the stopping residual,
+# 2.3e-11 against 5.7e-11, moves with the last bits of A), IR 5.4e-14.  The
+# tet bound is the stopping tolerance, the hex bound twice the measured error.
 HEX_ERR_BOUND = 2e-8
-TET_ERR_BOUND = 3e-11
+TET_ERR_BOUND = 1e-10
 
 
 def _scenarios():
@@ -63,6 +65,14 @@ def _scenarios():
     out["hex_ng4"].cond = np.tile((T / 1e-4).ravel(), (out["hex_ng4"].mesh.n_elem, 1)) * \
         out["hex_ng4"].cond[:, None]
     out["hex_ng4"].cond_stride = 9
+    # 3-D flow with a source and a nonzero start: the right-hand side does
+    # not live on the Dirichlet rows alone, so the reference's BiCGStab does
+    # not break down and hexes get BiCGStab parity too
+    hs = W.hex_flow(7, seed=19, n_transient=2, steady=False)
+    hs.st_idx = np.array([hs.mesh.node_index(3, 4, 3)], np.int64)
+    hs.st_val = np.array([-2e-4])
+    hs.u0 = np.random.default_rng(19).uniform(0.0, 1.0, hs.n)
+    out["hex_sink"] = hs
     # geo_area != 1 multiplies |J| of elements of dimension < 3
     # (SetMaterial, fem_ele_std.cpp:948-956): a 2-D flow with a full tensor
     # and gravity on the second axis, a 2-D heat transport with advection,
@@ -357,7 +367,8 @@ def test_reference_solution_chain_is_exact():
 
 
 @pytest.mark.parametrize("name", ["quad", "hex", "tet", "hex_aniso_grav", "quad_gw", "hex_ng4",
-                                  "quad_ng1", "quad_geo", "quad_heat_geo", "line_geo"])
+                                  "quad_ng1", "quad_geo", "quad_heat_geo", "line_geo",
+                                  "hex_sink"])
 @pytest.mark.parametrize("method,bc,prec,ir_mode", [
     (ox.CG, ox.DIRICHLET_SYMMETRIC, ox.FP64, None),
     (ox.BICGSTAB, ox.DIRICHLET_ROW, ox.FP64, None),
