@@ -200,6 +200,49 @@ def spmv_mode(size):
                       "tuning": s.spmv_tuning()}))
 
 
+def assembly_mode():
+    """Assembly alone, for kernel traces and hardware-counter passes: a 1 GiB
+    copy (byte-counter calibration), then per system one warm-up and 10
+    timed assemblies (Initialize + element pass + row pass, HIP events on
+    the system's stream) of the 128^3 hex flow system (transient step), the
+    96^3 x 6 tet heat system (advection, Crank-Nicolson) and the 512^2 quad
+    flow system.  Algorithmic bytes per assembly: per element its
+    connectivity, coefficients and (heat) Gauss-point velocities read once,
+    per node its coordinates and u_old read once, A and b written once."""
+    import ogs5_amd as ox
+    from ogs5_amd import workloads as W
+    out = {"copy_1GiB_gbs": ox._lib.stream_copy_gbs(1 << 30, 5)}
+    for name, sc in (("hex128_flow", W.hex_flow(128, n_transient=1)),
+                     ("tet96_heat", W.tet_heat(96, n_steps=1)),
+                     ("quad512_flow", W.quad_flow(512, 512, n_steps=1))):
+        pat, s = W.make_system(sc)
+        st = sc.steps[-1]
+        s.set_vector(ox.VEC_UOLD, sc.u0)
+        s.initialize()
+        s.assemble(st.dt, st.theta, False)
+        s.synchronize()
+        s.profile(True)
+        for _ in range(10):
+            s.initialize()
+            s.assemble(st.dt, st.theta, False)
+        ms, launches, _ = s.profile_read(ox._lib.PROF_ASSEMBLY)
+        s.profile(False)
+        m = sc.mesh
+        npe = m.conn.shape[1]
+        per_elem = 4 * npe + 8 + 8 * sc.cond_stride
+        if sc.vel is not None:
+            per_elem += 8 * sc.vel[0].size
+        alg = m.n_elem * per_elem + sc.n * (8 * m.dim + 8) + 8 * s.nnz + 8 * sc.n
+        per = ms / max(launches, 1)
+        out[name] = dict(elements=int(m.n_elem), nodes=int(sc.n), nnz=int(s.nnz),
+                         ms_per_assembly=per, algorithmic_bytes=int(alg),
+                         gbs=alg / per / 1e6)
+        _log(f"assembly {name}: {1e3 * per:.0f} us per assembly, {alg / per / 1e6:.0f} GB/s "
+             f"algorithmic")
+        s.close()
+    print(json.dumps(out))
+
+
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -595,7 +638,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["full", "spmv", "ensemble", "serve", "coupled", "heat", "quad"],
+    ap.add_argument("--mode", choices=["full", "spmv", "ensemble", "serve", "coupled", "heat", "quad",
+                                       "assembly"],
                     default="full")
     ap.add_argument("--coupled-size", type=int, default=64)
     ap.add_argument("--heat-size", type=int, default=96)
@@ -619,6 +663,9 @@ def main():
         sys.exit(2)
     if args.mode == "spmv":
         spmv_mode(args.size)
+        return
+    if args.mode == "assembly":
+        assembly_mode()
         return
     if args.mode == "ensemble":
         ensemble_mode(args.instances, args.ens_steps)

@@ -184,7 +184,8 @@ struct ogsx_system {
     // allocated on the first assembly that uses them
     int *inc_ptr = nullptr, *inc_ele = nullptr, *asm_pos = nullptr;
     uint8_t* asm_loc = nullptr;
-    double* asm_eb = nullptr;  // records [E*npe][npe + 1 or npe + 2]
+    double* asm_eb = nullptr;  // element rows [E*npe][npe], then right-hand sides
+                               // and gravity terms [E*npe][2]
     // values
     double* A = nullptr;
     float* A32 = nullptr;
@@ -1207,7 +1208,7 @@ void launch_assemble_colours(ogsx_system* s, const AsmArgs& a)
         if (h.n_elem > 0)
             k_assemble<ET, PROC, ISO, true><<<(int)((h.n_elem + 63) / 64), 64, 0, s->stream>>>(
                 a, 0, (int)h.n_elem);
-        const bool grav = a.rs == ETraits<ET>::NPE + 2;
+        const bool grav = PROC == OGSX_LIQUID_FLOW && a.rho_g != 0.0;
         const int lanes = asm_row_lanes(h.row_max);
         const int blocks = (int)(((int64_t)s->n * lanes + kAsmRowThreads - 1) / kAsmRowThreads);
         if (blocks == 0) return;
@@ -1412,8 +1413,8 @@ AsmArgs asm_args(const ogsx_system* s, double dt, double theta, int steady)
     a.theta = theta;
     if (s->asm_eb) {  // two-pass assembly (buffers from ogsx_system_set_elements)
         a.eb = s->asm_eb;
+        a.rb = s->asm_eb + s->pat->n_elem * s->pat->npe * s->pat->npe;
         a.pos = s->asm_pos;
-        a.rs = s->pat->npe + (asm_args_gravity(s) ? 2 : 1);
     }
     return a;
 }

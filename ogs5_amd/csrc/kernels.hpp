@@ -722,15 +722,16 @@ struct AsmArgs {
     const double* u_old;
     double* A;
     double* b;
-    double* eb;         // two-pass assembly: one record per (element, node)
-                        // pair at its incidence position pos: the element's
-                        // row, its right-hand side and, with gravity, its
-                        // gravity term (rs = npe + 1 or npe + 2 doubles)
+    double* eb;         // two-pass assembly, per (element, node) pair at its
+                        // incidence position q = pos[e npe + i]: the
+                        // element's row i at eb[q npe], its right-hand side
+                        // and gravity term (0 without gravity) at rb[2 q]
+    double* rb;
     const int32_t* pos;
     int cond_stride;
     int ngp;
     int steady;
-    int rs;             // record stride (doubles)
+    int pad;
     double tuf, mu, geo_area, adv_coef, rho_g, dt, theta;
 };
 
@@ -989,11 +990,11 @@ __device__ __forceinline__ void assemble_elem(const AsmArgs& a, int e)
 #pragma unroll
         for (int i = 0; i < NPE; i++) {
             const int64_t q = a.pos[(int64_t)e * NPE + i];
-            double* rec = a.eb + q * a.rs;
+            double* rec = a.eb + q * NPE;
 #pragma unroll
             for (int j = 0; j < NPE; j++) rec[j] = lhs[i * NPE + j];
-            rec[NPE] = rhs[i];
-            if (do_grav) rec[NPE + 1] = G[i];
+            a.rb[2 * q] = rhs[i];  // one 16-byte store with the gravity term
+            a.rb[2 * q + 1] = do_grav ? G[i] : 0.0;
         }
         return;
     }
@@ -1025,12 +1026,11 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a, int off, int cnt)
 // record whose column position (loc) is c.  Every entry is therefore summed
 // from A's current value in the same order as A += contribution element by
 // element, and no two lanes touch one entry (no colouring, no atomics).  The
-// row's lanes read each record together (one contiguous stretch per row), so
-// a wave holds a few cache lines at a time and each line is fetched once;
-// a lane issues the loads of 8 records before it adds them, so a row costs a
-// few memory round trips rather than two per record.  Lane 0 adds the
-// element right-hand sides, then the gravity terms, to b[r] as the scatter
-// does.
+// row's lanes load its records together, one record per lane, through LDS
+// (one contiguous stretch per row), so a wave holds a few cache lines at a
+// time, each line is fetched once, and a row costs one memory round trip per
+// G records.  Lane 0 adds the element right-hand sides, then the gravity
+// terms, to b[r] as the scatter does.
 constexpr int kAsmRowMax = 32;      // longest row the two-pass assembly takes
 constexpr int kAsmRowThreads = 256; // threads per block of the row pass
 
@@ -1052,51 +1052,79 @@ __device__ __forceinline__ int loc_match(uint64_t w, int c)
     return jm;
 }
 
-constexpr int kAsmRowBatch = 8;  // records whose loads a lane issues together
+// LDS written by some lanes of a wavefront and read by others of the same
+// wavefront: the wavefront's LDS accesses complete in order once waited for.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-template <int ET, bool GRAV>
+// The row's records are taken G at a time: lane t loads record t of the
+// batch (a contiguous stretch of the row's records across the lanes) and its
+// column positions into LDS, then lane c walks the batch's records in order.
+// The lanes of a row sit in one wavefront; every lane of a wavefront runs
+// every batch any of its rows needs (no early exit), so the wavefront-level
+// syncs are uniform.  lds holds kAsmRowThreads records, ldw as many words.
+template <int ET, bool GRAV, int G>
 __device__ __forceinline__ void assemble_row(const AsmArgs& a, const int32_t* inc_ptr,
-                                             const uint8_t* loc, const int32_t* ptr, int r, int c)
+                                             const uint8_t* loc, const int32_t* ptr, int n,
+                                             int64_t r, double* lds, uint64_t* ldw)
 {
     constexpr int NPE = ETraits<ET>::NPE;
     constexpr int RS = NPE + (GRAV ? 2 : 1);
-    constexpr int B = kAsmRowBatch;
     using W = LocWord<NPE>;
+    static_assert(64 % G == 0, "a row's lanes must sit in one wavefront");
+    const int tid = threadIdx.x;
+    const int c = tid % G, g0 = tid - c;
+    const bool live = r < n;
     const OGSX_GL double* eb = gl(a.eb);
+    const OGSX_GL double* rb = gl(a.rb);
     const OGSX_GL W* lw = (const OGSX_GL W*)gl(loc);
     OGSX_GL double* A = gl(a.A);
-    const int p0 = ptr[r], len = ptr[r + 1] - p0;
-    const bool mine = c < len;
-    double acc = mine ? A[p0 + c] : 0.0;
-    double br = c == 0 ? gl(a.b)[r] : 0.0;
-    const int64_t q0 = inc_ptr[r], q1 = inc_ptr[r + 1];
-    for (int64_t q = q0; q < q1; q += B) {
-        // all loads of the batch first, then the sums in record order
-        uint64_t w[B];
+    int p0 = 0, len = 0;
+    int64_t q0 = 0, q1 = 0;
+    double acc = 0.0, br = 0.0;
+    if (live) {
+        p0 = ptr[r];
+        len = ptr[r + 1] - p0;
+        q0 = inc_ptr[r];
+        q1 = inc_ptr[r + 1];
+        if (c < len) acc = A[p0 + c];
+        if (c == 0) br = gl(a.b)[r];
+    }
+    for (int64_t base = 0; __any(live && q0 + base < q1); base += G) {
+        const int64_t q = q0 + base + c;
+        if (live && q < q1) {
+            double v[RS];
 #pragma unroll
-        for (int k = 0; k < B; k++) w[k] = q + k < q1 ? (uint64_t)lw[q + k] : ~0ull;
-        double v[B], rb[B], gb[B];
-        int jm[B];
+            for (int j = 0; j < NPE; j++) v[j] = eb[q * NPE + j];
+            v[NPE] = rb[2 * q];
+            if (GRAV) v[NPE + 1] = rb[2 * q + 1];
+            const uint64_t w = lw[q];
 #pragma unroll
-        for (int k = 0; k < B; k++) {
-            jm[k] = mine ? loc_match<NPE>(w[k], c) : -1;
-            const OGSX_GL double* rec = eb + (q + k) * RS;
-            v[k] = jm[k] >= 0 ? rec[jm[k]] : 0.0;
-            const bool rh = c == 0 && q + k < q1;
-            rb[k] = rh ? rec[NPE] : 0.0;
-            gb[k] = GRAV && rh ? rec[NPE + 1] : 0.0;
+            for (int j = 0; j < RS; j++) lds[tid * RS + j] = v[j];
+            ldw[tid] = w;
         }
-#pragma unroll
-        for (int k = 0; k < B; k++) {
-            if (jm[k] >= 0) acc += v[k];
-            if (c == 0 && q + k < q1) {
-                br += rb[k];
-                if (GRAV) br += gb[k];
+        wave_lds_sync();
+        const int64_t rem = q1 - q0 - base;
+        const int cnt = live && rem > 0 ? (int)(rem < G ? rem : G) : 0;
+        for (int k = 0; k < cnt; k++) {
+            const double* rec = lds + (g0 + k) * RS;
+            const int jm = c < len ? loc_match<NPE>(ldw[g0 + k], c) : -1;
+            if (jm >= 0) acc += rec[jm];
+            if (c == 0) {
+                br += rec[NPE];
+                if (GRAV) br += rec[NPE + 1];
             }
         }
+        wave_lds_sync();
     }
-    if (mine) A[p0 + c] = acc;
-    if (c == 0) gl(a.b)[r] = br;
+    if (live) {
+        if (c < len) A[p0 + c] = acc;
+        if (c == 0) gl(a.b)[r] = br;
+    }
 }
 
 template <int ET, bool GRAV, int G>
@@ -1104,9 +1132,10 @@ __global__ __launch_bounds__(kAsmRowThreads) void k_assemble_rows(AsmArgs a, con
                                                                   const uint8_t* loc,
                                                                   const int32_t* ptr, int n)
 {
-    const int r = (int)((blockIdx.x * (int64_t)kAsmRowThreads + threadIdx.x) / G);
-    if (r >= n) return;
-    assemble_row<ET, GRAV>(a, inc_ptr, loc, ptr, r, (int)threadIdx.x % G);
+    __shared__ double lds[kAsmRowThreads * (ETraits<ET>::NPE + (GRAV ? 2 : 1))];
+    __shared__ uint64_t ldw[kAsmRowThreads];
+    const int64_t r = ((int64_t)blockIdx.x * kAsmRowThreads + threadIdx.x) / G;
+    assemble_row<ET, GRAV, G>(a, inc_ptr, loc, ptr, n, r, lds, ldw);
 }
 
 // Two-pass assembly tables, per incidence position q (node-major, inc_ei[q] =
@@ -2879,12 +2908,13 @@ __global__ __launch_bounds__(64) void k_step_assemble(StepMap m)
 template <int ET, bool GRAV, int G>
 __global__ __launch_bounds__(kAsmRowThreads) void k_step_rows(StepMap m)
 {
+    __shared__ double lds[kAsmRowThreads * (ETraits<ET>::NPE + (GRAV ? 2 : 1))];
+    __shared__ uint64_t ldw[kAsmRowThreads];
     const StepCtx c = step_ctx(m);
     const StepArgs& S = *c.S;
-    const int r = (int)((c.lb * (int64_t)kAsmRowThreads + threadIdx.x) / G);
-    if (r >= S.n) return;
+    const int64_t r = (c.lb * (int64_t)kAsmRowThreads + threadIdx.x) / G;
     const AsmArgs a = S.a;
-    assemble_row<ET, GRAV>(a, S.inc_ptr, S.loc, S.ptr, r, (int)threadIdx.x % G);
+    assemble_row<ET, GRAV, G>(a, S.inc_ptr, S.loc, S.ptr, S.n, r, lds, ldw);
 }
 
 // b[node] += the node's source values in list order (as k_add_rhs)
