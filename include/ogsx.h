@@ -210,7 +210,9 @@ typedef struct {
     /* OGSX_FP32_IR systems only */
     double inner_tol;  /* OGSX_IR_RESTART: relative reduction asked of each
                           FP32 inner solve; OGSX_IR_RELIABLE: the drop delta
-                          of the FP32 residual that triggers an FP64 update */
+                          of the FP32 residual that triggers an FP64 update;
+                          <= 0: the default (0.1 reliable; restarted 1e-4
+                          for CG, 1e-3 for BiCGStab)                        */
     int max_outer;     /* refinement steps (OGSX_IR_RESTART)                  */
     int ir_mode;       /* OGSX_IR_RESTART or OGSX_IR_RELIABLE (CG only)       */
 } ogsx_solver_config;

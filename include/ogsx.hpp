@@ -88,9 +88,10 @@ public:
     // ConfigNumerics (equation_class.cpp:191): $LINEAR_SOLVER method, precond,
     // tolerance, max iterations.  symmetric_dirichlet selects the symmetric
     // elimination that CG needs (the reference's row-only form otherwise).
+    // inner_tol 0 takes the library's default for the refinement scheme.
     void ConfigNumerics(int ls_method, int ls_precond, double ls_error_tolerance,
                         int ls_max_iterations, bool symmetric_dirichlet = false,
-                        double inner_tol = 1e-4, int max_outer = 50,
+                        double inner_tol = 0.0, int max_outer = 50,
                         int ir_mode = OGSX_IR_RESTART)
     {
         cfg_.method = ls_method;

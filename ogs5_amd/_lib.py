@@ -169,11 +169,13 @@ def _i64(a):
 def solver_config(method=CG, precond=PRECOND_JACOBI, tol=1e-10, max_iter=10000,
                   inner_tol=None, max_outer=50, ir_mode=None) -> SolverConfig:
     """ogsx_solver_config; for IR systems CG defaults to reliable updates
-    (inner_tol = delta 0.1), BiCGStab to restarted refinement (1e-4)."""
+    (inner_tol = delta 0.1), BiCGStab to restarted refinement (inner
+    tolerance 1e-3; 1e-4 for restarted CG)."""
     if ir_mode is None:
         ir_mode = IR_RELIABLE if method == CG else IR_RESTART
     if inner_tol is None:
-        inner_tol = 0.1 if ir_mode in (IR_RELIABLE, IR_RELIABLE_SR) else 1e-4
+        inner_tol = (0.1 if ir_mode in (IR_RELIABLE, IR_RELIABLE_SR)
+                     else 1e-3 if method == BICGSTAB else 1e-4)
     return SolverConfig(method, precond, max_iter, tol, inner_tol, max_outer, ir_mode)
 
 
