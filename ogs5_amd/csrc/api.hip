@@ -185,7 +185,8 @@ struct ogsx_system {
     int *inc_ptr = nullptr, *inc_ele = nullptr, *asm_pos = nullptr;
     uint8_t* asm_loc = nullptr;
     double* asm_eb = nullptr;  // element rows [E*npe][npe], then right-hand sides
-                               // and gravity terms [E*npe][2]
+                               // [E*npe][rb_stride] (the second of a pair is
+                               // the gravity term or 0)
     // values
     double* A = nullptr;
     float* A32 = nullptr;
