@@ -61,6 +61,13 @@ int grid_for(int64_t n)
     return (int)std::max<int64_t>(kNumXcd, std::min<int64_t>(g, 2048));
 }
 
+// Whether some of the G row chunks of an n-row system hold fewer than the
+// 4 kBlock rows of one full batch of the CG vector kernels (their SMALL form).
+bool chunk_rows_small(int64_t n, int G)
+{
+    return n / G < 4 * kBlock;
+}
+
 int blocks_for(int64_t n, int per)
 {
     int64_t b = (n + per - 1) / per;
