@@ -172,6 +172,29 @@ __device__ __forceinline__ void vec_chunk(int n, int G, int b, Body&& body)
     for (int i = beg + threadIdx.x; i < end; i += kBlock) body(i);
 }
 
+// The rows of chunk b, kVecU per thread at a time: load(k, u) reads row k
+// into slot u for all kVecU slots (rows past the chunk's end read its last
+// row and are not used), then apply(k, u) runs for the slots inside the
+// chunk, in ascending row order.  A thread thus waits for memory once per
+// batch instead of once (or, where stores may alias the next row's loads,
+// several times) per row; per-thread order, hence every sum, is that of
+// vec_chunk.
+constexpr int kVecU = 4;
+template <typename Load, typename Apply>
+__device__ __forceinline__ void vec_chunk_batched(int n, int G, int b, Load&& load, Apply&& apply)
+{
+    int beg, end;
+    chunk_of(n, G, b, beg, end);
+    const int last = end - 1;
+    for (int i = beg + threadIdx.x; i < end; i += kVecU * kBlock) {
+#pragma unroll
+        for (int u = 0; u < kVecU; u++) load(min(i + u * kBlock, last), u);
+#pragma unroll
+        for (int u = 0; u < kVecU; u++)
+            if (i + u * kBlock < end) apply(i + u * kBlock, u);
+    }
+}
+
 // Matrix as the SpMV kernels see it: CRS (ptr, col, val) plus the row tiles
 // of the tiled format (tile_row: first row of every tile; tile_cb: the
 // smallest column of the tile, col16 = col - tile_cb when it fits 16 bits).
@@ -2004,20 +2027,35 @@ __device__ __forceinline__ void bi_dir_body(const Grp& g, int vb)
         return;
     }
     const GVec<T> v = gvec(kv_of<T>(S));
+    T rr[kVecU], pp[kVecU], vv[kVecU], dd[kVecU];
     if (it == 0) {
-        vec_vblocks(c, S.n, [&](int i) {
-            const T pi = v.r[i];
-            v.p[i] = pi;
-            v.ph[i] = pi / v.d[i];
-        });
+        vec_chunk_batched(
+            S.n, c.G, c.b,
+            [&](int k, int u) {
+                rr[u] = v.r[k];
+                dd[u] = v.d[k];
+            },
+            [&](int k, int u) {
+                const T pi = rr[u];
+                v.p[k] = pi;
+                v.ph[k] = pi / dd[u];
+            });
     } else {
         const T beta = (T)((rho1 / st->rho0) * (st->alpha / st->omega));
         const T omega = (T)st->omega;
-        vec_vblocks(c, S.n, [&](int i) {
-            const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
-            v.p[i] = pi;
-            v.ph[i] = pi / v.d[i];
-        });
+        vec_chunk_batched(
+            S.n, c.G, c.b,
+            [&](int k, int u) {
+                rr[u] = v.r[k];
+                pp[u] = v.p[k];
+                vv[u] = v.v[k];
+                dd[u] = v.d[k];
+            },
+            [&](int k, int u) {
+                const T pi = rr[u] + beta * (pp[u] - omega * vv[u]);
+                v.p[k] = pi;
+                v.ph[k] = pi / dd[u];
+            });
     }
     if (c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
 }
@@ -2073,12 +2111,20 @@ __device__ __forceinline__ void bi_s_body(const Grp& g, int vb)
     {
         const int b = c.b;
         double acc[1] = {0.0};
-        vec_chunk(S.n, c.G, b, [&](int i) {
-            const T si = v.r[i] - alpha * v.v[i];
-            v.s[i] = si;
-            v.sh[i] = si / v.d[i];
-            acc[0] += (double)si * (double)si;
-        });
+        T rr[kVecU], vv[kVecU], dd[kVecU];
+        vec_chunk_batched(
+            S.n, c.G, b,
+            [&](int k, int u) {
+                rr[u] = v.r[k];
+                vv[u] = v.v[k];
+                dd[u] = v.d[k];
+            },
+            [&](int k, int u) {
+                const T si = rr[u] - alpha * vv[u];
+                v.s[k] = si;
+                v.sh[k] = si / dd[u];
+                acc[0] += (double)si * (double)si;
+            });
         store_partials<1>(acc, gl(S.part) + 8 * (size_t)c.G, c.G, b);
     }
     if (c.b == 0 && threadIdx.x == 0) {
@@ -2152,13 +2198,24 @@ __device__ __forceinline__ void bi_update_body(const Grp& g, int vb)
     {
         const int b = c.b;
         double acc[2] = {0.0, 0.0};
-        vec_chunk(S.n, c.G, b, [&](int i) {
-            v.x[i] += alpha * v.ph[i] + omega * v.sh[i];
-            const T ri = v.s[i] - omega * v.t[i];
-            v.r[i] = ri;
-            acc[0] += (double)ri * (double)ri;
-            acc[1] += (double)v.r0[i] * (double)ri;
-        });
+        T xx[kVecU], ph[kVecU], sh[kVecU], ss[kVecU], tt[kVecU], r0[kVecU];
+        vec_chunk_batched(
+            S.n, c.G, b,
+            [&](int k, int u) {
+                xx[u] = v.x[k];
+                ph[u] = v.ph[k];
+                sh[u] = v.sh[k];
+                ss[u] = v.s[k];
+                tt[u] = v.t[k];
+                r0[u] = v.r0[k];
+            },
+            [&](int k, int u) {
+                v.x[k] = xx[u] + (alpha * ph[u] + omega * sh[u]);
+                const T ri = ss[u] - omega * tt[u];
+                v.r[k] = ri;
+                acc[0] += (double)ri * (double)ri;
+                acc[1] += (double)r0[u] * (double)ri;
+            });
         store_partials<2>(acc, gl(S.part), c.G, b);
     }
     if (c.b == 0 && threadIdx.x == 0) st->omega = omega_d;
