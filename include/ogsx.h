@@ -359,7 +359,7 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
 #define OGSX_PROF_ASSEMBLY 2
 #define OGSX_PROF_VELOCITY 3 /* ogsx_gauss_velocity kernel (on the flow system) */
 /* SpMV storage/launch variant: 0 = autotune (time variants 3, 5, 6, 7, 10,
- * 11, 12, 13, 14, 17, 18, 19, 20 on this matrix at first use and keep the
+ * 11, 12, 13, 14, 17, 18, 21, 22, 19, 20 on this matrix at first use and keep the
  * fastest; all of them sum each row in CRS order over rounded products,
  * bit-identical to CSparseMatrix::multiVec, and hand each row to the same
  * thread of the same block, so the dot products the solve fuses into its
@@ -376,7 +376,9 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * row, columns = row + the pattern's offsets; when the pattern has at most
  * 255 distinct offset lists), 8 / 7 entries in flight, non-temporal; 17 / 18
  * = as 13 / 14 with cached matrix loads (a matrix that fits the caches stays
- * there between launches); 19 / 20 = SELL-64 with one stencil per slice
+ * there between launches); 21 / 22 = as 13 / 17 taking a row's entries 16
+ * at a time with the last batch predicated (rows of up to 16 entries in one
+ * round trip); 19 / 20 = SELL-64 with one stencil per slice
  * in triples of consecutive column offsets (one coalesced x load per triple,
  * its +1 / +2 neighbours by lane shifts; zero entries where a row lacks a
  * column of its slice's stencil; when every slice needs at most 16 triples

@@ -297,7 +297,8 @@ int dalloc(ogsx_system* s, T** p, size_t count)
 
 inline bool is_pat(int f)
 {
-    return f == kFmtSellPat8 || f == kFmtSellPat7 || f == kFmtSellPat8C || f == kFmtSellPat7C;
+    return f == kFmtSellPat8 || f == kFmtSellPat7 || f == kFmtSellPat8C || f == kFmtSellPat7C ||
+           f == kFmtSellPat16 || f == kFmtSellPat16C;
 }
 
 inline bool is_tri(int f)
@@ -712,6 +713,8 @@ int prof_flush(ogsx_system* s)
         case kFmtSellPat7: { constexpr int L = kFmtSellPat7; CALL; } break; \
         case kFmtSellPat8C: { constexpr int L = kFmtSellPat8C; CALL; } break; \
         case kFmtSellPat7C: { constexpr int L = kFmtSellPat7C; CALL; } break; \
+        case kFmtSellPat16: { constexpr int L = kFmtSellPat16; CALL; } break; \
+        case kFmtSellPat16C: { constexpr int L = kFmtSellPat16C; CALL; } break; \
         case kFmtTri5: { constexpr int L = kFmtTri5; CALL; } break;           \
         case kFmtTri4C: { constexpr int L = kFmtTri4C; CALL; } break;         \
         case kFmtCrs1: { constexpr int L = kFmtCrs1; CALL; } break;           \
@@ -753,8 +756,8 @@ int tune_one(ogsx_system* s, int* fmt_out)
     // by hand, accumulate rows in another order)
     const int cands[] = {kFmtSell16,    kFmtSell16U8, kFmtSell16NT, kFmtSell16U16,
                          kFmtSell16X2,  kFmtSell16U7, kFmtSellPat8, kFmtSellPat7,
-                         kFmtSellPat8C, kFmtSellPat7C, kFmtTri5,    kFmtTri4C,
-                         kFmtSell32};
+                         kFmtSellPat8C, kFmtSellPat7C, kFmtSellPat16, kFmtSellPat16C,
+                         kFmtTri5,      kFmtTri4C,    kFmtSell32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -291,6 +291,10 @@ constexpr int kFmtSellPat8 = 13;          // SELL-64, stencil dictionary, 8 in f
 constexpr int kFmtSellPat7 = 14;          // SELL-64, stencil dictionary, 7 in flight, NT
 constexpr int kFmtSellPat8C = 17;         // ... 8 in flight, cached loads
 constexpr int kFmtSellPat7C = 18;         // ... 7 in flight, cached loads
+constexpr int kFmtSellPat16 = 21;         // ... a row's entries 16 at a time, the last
+                                          // batch predicated (one round trip for
+                                          // rows of up to 16 entries), NT
+constexpr int kFmtSellPat16C = 22;        // ... the same with cached loads
 constexpr int kSellC = 64;                // rows per slice = wavefront size
 constexpr int kPatLds = 4096;             // dictionary entries staged in LDS
 constexpr int kFmtTri5 = 19;              // SELL-64 slice triples, 5 in flight, NT
@@ -460,7 +464,14 @@ __device__ __forceinline__ void spmv_sell(const GMat<VT>& m, const OGSX_GL XT* _
 // those of the other SELL formats (same value array, CRS order, rounded
 // products), so results are bit-identical.  Padding entries have value 0
 // and delta 0; lanes past the last row take the last row's column.
-template <int U, bool NT, typename VT, typename XT, typename ACC, typename Epi>
+//
+// PRED: the row's entries are taken U at a time down to the last, partial
+// batch, whose missing entries read the row's last entry again and are not
+// added (the slice width w is the same for the wave, so the test is
+// uniform).  With U = 16 a row of up to 16 entries (2-D quads 9, tets 15)
+// costs one round trip for its values and one for its x gathers, where the
+// other forms add the entries past the last full batch one at a time.
+template <int U, bool NT, bool PRED, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_sell_pat(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
                                               int G, int b, Epi&& epi)
 {
@@ -479,6 +490,30 @@ __device__ __forceinline__ void spmv_sell_pat(const GMat<VT>& m, const OGSX_GL X
         const int w = m.sell_w[sl];
         const int* pr = pd + (int)m.sell_pid[row] * m.pat_w;
         ACC s = 0;
+        if constexpr (PRED) {
+            for (int j = 0; j < w; j += U) {
+                int c[U];
+                VT v[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int jj = min(j + u, w - 1);
+                    v[u] = ld<NT>(m.sval + base + jj * kSellC);
+                    c[u] = rowc + pr[jj];
+                }
+                ACC xv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) xv[u] = (ACC)x[c[u]];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (j + u < w) {
+                        const ACC prod = (ACC)v[u] * xv[u];
+                        s = s + prod;
+                    }
+                }
+            }
+            if (row < m.n) epi(row, s);
+            continue;
+        }
         int j = 0;
         for (; j + U <= w; j += U) {
             int c[U];
@@ -707,13 +742,17 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
     else if constexpr (FMT == kFmtSell16U7)
         spmv_sell<1, 7, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSellPat8)
-        spmv_sell_pat<8, true, VT, XT, ACC>(m, x, G, b, epi);
+        spmv_sell_pat<8, true, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSellPat7)
-        spmv_sell_pat<7, true, VT, XT, ACC>(m, x, G, b, epi);
+        spmv_sell_pat<7, true, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSellPat8C)
-        spmv_sell_pat<8, false, VT, XT, ACC>(m, x, G, b, epi);
+        spmv_sell_pat<8, false, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSellPat7C)
-        spmv_sell_pat<7, false, VT, XT, ACC>(m, x, G, b, epi);
+        spmv_sell_pat<7, false, false, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSellPat16)
+        spmv_sell_pat<16, true, true, VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSellPat16C)
+        spmv_sell_pat<16, false, true, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtSell16X2)
         spmv_sell_x2<4, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtTri5)

@@ -196,7 +196,7 @@ def test_assembly_parity_long_rows(process, bc):
     s.close()
 
 
-@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 9, 10, 11, 12, 13, 14, 17, 18, 19, 20,
+@pytest.mark.parametrize("lpr", [1, 2, 3, 5, 6, 7, 9, 10, 11, 12, 13, 14, 17, 18, 21, 22, 19, 20,
                                  4, 8, 16, 32, 64])
 def test_spmv_parity_all_variants(lpr):
     sc = SCEN["hex"]
@@ -207,7 +207,7 @@ def test_spmv_parity_all_variants(lpr):
     assert s.spmv_variant() == lpr
     eq.a[:] = A  # same matrix on both sides
     yr = eq.spmv(x)
-    if lpr in (1, 2, 3, 5, 6, 7, 9, 10, 11, 12, 13, 14, 17, 18, 19, 20):
+    if lpr in (1, 2, 3, 5, 6, 7, 9, 10, 11, 12, 13, 14, 17, 18, 21, 22, 19, 20):
         # CRS-order summation over rounded products: the reference's multiVec bits
         assert np.array_equal(y, yr)
         return
@@ -218,7 +218,7 @@ def test_spmv_parity_all_variants(lpr):
 
 
 @pytest.mark.parametrize("name", ["quad", "tet", "hex_aniso_grav", "bmskel"])
-@pytest.mark.parametrize("fmt", [13, 14, 17, 18])
+@pytest.mark.parametrize("fmt", [13, 14, 17, 18, 21, 22])
 def test_spmv_stencil_dictionary_bit_exact(name, fmt):
     """The dictionary formats on 2-D, tet and small meshes (padding rows in
     the last slice): the reference's multiVec bits."""
@@ -282,7 +282,7 @@ def test_solves_identical_in_every_tuned_format(name, method, bc):
     bit-identical whichever format the timing-based tuner chose."""
     sc = SCEN[name]
     ref = None
-    for fmt in (9, 3, 5, 7, 11, 12, 13, 14, 17, 18, 19, 20):
+    for fmt in (9, 3, 5, 7, 11, 12, 13, 14, 17, 18, 21, 22, 19, 20):
         pat, s = W.make_system(sc)
         try:
             s.set_spmv_variant(fmt)
@@ -509,7 +509,7 @@ def test_full_size_tet_heat_properties():
     s.set_spmv_variant(5)
     y5 = s.spmv(x)
     tried = []
-    for f in (3, 13, 14, 17, 18, 19, 20):
+    for f in (3, 13, 14, 17, 18, 21, 22, 19, 20):
         try:
             s.set_spmv_variant(f)
         except ox.OgsxError:
