@@ -42,7 +42,8 @@ int fail(int code, const std::string& msg)
 // Blocks of the n-sized kernels of one system (each block a contiguous chunk
 // of rows).  G sets the partial-sum layout, so it is a function of n alone,
 // never tuned at run time: one row per thread up to OGSX_FILL_BLOCKS blocks,
-// then OGSX_ROWS_PER_BLOCK rows per block, at most 2048 blocks.
+// then OGSX_ROWS_PER_BLOCK rows per block, at most kMaxGrid (2048) blocks
+// (the partial sums' reduction loads all of them in one batch).
 // Few rows per block keep mid-size systems (latency-bound) spread over the
 // CUs; many rows per block cut block dispatches and the per-block reduction
 // of partial sums for large systems and for groups of many systems.
@@ -58,7 +59,7 @@ int grid_for(int64_t n)
     int64_t g = (n + OGSX_ROWS_PER_BLOCK - 1) / OGSX_ROWS_PER_BLOCK;
     g = std::max<int64_t>(g, std::min<int64_t>((n + kBlock - 1) / kBlock, OGSX_FILL_BLOCKS));
     g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
-    return (int)std::max<int64_t>(kNumXcd, std::min<int64_t>(g, 2048));
+    return (int)std::max<int64_t>(kNumXcd, std::min<int64_t>(g, kMaxGrid));
 }
 
 // Whether some of the G row chunks of an n-row system hold fewer than the

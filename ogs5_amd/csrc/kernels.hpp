@@ -82,19 +82,27 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], OGSX_GL double* 
 }
 
 // Sums G partials per value in a fixed order; every thread gets the result.
-// G is the grid size of the producer, which equals this kernel's grid.
-// Thread t adds partials t, t + kBlock, t + 2 kBlock, ... in that order.  The
-// first kPartialBatch of them per value are all loaded before the first add,
-// so a block waits for one round trip to L2 instead of one per partial (the
+// G is the grid size of the producer, which equals this kernel's grid, and
+// at most kMaxGrid (grid_for's cap).  Thread t adds partials t, t + kBlock,
+// t + 2 kBlock, ... in that order, all loaded before the first add, so a
+// block waits for one round trip to L2 instead of one per partial (the
 // partials were written by blocks on every XCD and miss the local L2).
-constexpr int kPartialBatch = 2048 / kBlock;
+constexpr int kMaxGrid = 2048;
+constexpr int kPartialBatch = kMaxGrid / kBlock;
 template <int NV>
-__device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, int G,
-                                                double (&out)[NV])
-{
-    __shared__ double sm[NV][kBlock / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+struct PartialLoads {
     double t[NV][kPartialBatch];
+};
+
+// The first kPartialBatch partials per value of thread t, loaded (not yet
+// added).  A kernel that also streams rows issues these loads first and its
+// rows' loads after them, so that the reduction can wait for the partials
+// (from L2) while the rows (from HBM) are still in flight: a wait on the
+// memory counter covers every load issued before the one awaited.
+template <int NV>
+__device__ __forceinline__ void load_partials(const OGSX_GL double* partials, int G,
+                                              PartialLoads<NV>& P)
+{
 #pragma unroll
     for (int k = 0; k < NV; k++)
 #pragma unroll
@@ -102,8 +110,19 @@ __device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, 
             // clamped index: every load is issued unpredicated (G >= 1)
             const int i = threadIdx.x + j * kBlock;
             const double x = partials[(size_t)k * G + (i < G ? i : G - 1)];
-            t[k][j] = i < G ? x : 0.0;
+            P.t[k][j] = i < G ? x : 0.0;
         }
+}
+
+// Completes reduce_partials from loads made by load_partials.
+template <int NV>
+__device__ __forceinline__ void finish_partials(const OGSX_GL double* partials, int G,
+                                                const PartialLoads<NV>& P, double (&out)[NV])
+{
+    __shared__ double sm[NV][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const auto& t = P.t;
+    (void)partials;
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -111,8 +130,6 @@ __device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, 
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < kPartialBatch; j++) s += t[k][j];
-        for (int i = threadIdx.x + kPartialBatch * kBlock; i < G; i += kBlock)
-            s += partials[(size_t)k * G + i];
         v[k] = wave_sum(s);
     }
     if (lane == 0)
@@ -127,6 +144,15 @@ __device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, 
         out[k] = s;
     }
     __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const OGSX_GL double* partials, int G,
+                                                double (&out)[NV])
+{
+    PartialLoads<NV> P;
+    load_partials<NV>(partials, G, P);
+    finish_partials<NV>(partials, G, P, out);
 }
 
 // Sum over one CRS row by a group of LPR lanes; every lane of the group gets
@@ -178,21 +204,35 @@ __device__ __forceinline__ void vec_chunk(int n, int G, int b, Body&& body)
 // chunk, in ascending row order.  A thread thus waits for memory once per
 // batch instead of once (or, where stores may alias the next row's loads,
 // several times) per row; per-thread order, hence every sum, is that of
-// vec_chunk.
+// vec_chunk.  mid() runs on every thread between the first batch's loads
+// and their use -- the reduction of the previous kernel's partials, whose
+// loads the caller issued first, so it overlaps the rows in flight; when it
+// returns false the rows are left alone.
 constexpr int kVecU = 4;
-template <typename Load, typename Apply>
-__device__ __forceinline__ void vec_chunk_batched(int n, int G, int b, Load&& load, Apply&& apply)
+
+template <typename Load, typename Mid, typename Apply>
+__device__ __forceinline__ bool vec_chunk_batched_mid(int n, int G, int b, Load&& load, Mid&& mid,
+                                                      Apply&& apply)
 {
     int beg, end;
     chunk_of(n, G, b, beg, end);
     const int last = end - 1;
-    for (int i = beg + threadIdx.x; i < end; i += kVecU * kBlock) {
+    int i = beg + threadIdx.x;
+    if (i < end) {
 #pragma unroll
         for (int u = 0; u < kVecU; u++) load(min(i + u * kBlock, last), u);
+    }
+    if (!mid()) return false;
+    for (bool loaded = true; i < end; i += kVecU * kBlock, loaded = false) {
+        if (!loaded) {
+#pragma unroll
+            for (int u = 0; u < kVecU; u++) load(min(i + u * kBlock, last), u);
+        }
 #pragma unroll
         for (int u = 0; u < kVecU; u++)
             if (i + u * kBlock < end) apply(i + u * kBlock, u);
     }
+    return true;
 }
 
 // Matrix as the SpMV kernels see it: CRS (ptr, col, val) plus the row tiles
@@ -1742,6 +1782,9 @@ __device__ __forceinline__ void cg_update_body(const Grp& g, int vb)
     int i = beg + threadIdx.x;
     // the first batch of the stream does not depend on alpha: its loads are
     // issued before the partials are reduced, so their latency overlaps
+    // partials first, then the first batch of rows (see load_partials)
+    PartialLoads<1> pl;
+    load_partials<1>(gl(S.part), c.G, pl);
     T xi[U], pi[U], ri[U], si[U], di[U];
     const bool first = i + (U - 1) * kBlock < end;
     if (first) {
@@ -1766,7 +1809,7 @@ __device__ __forceinline__ void cg_update_body(const Grp& g, int vb)
         }
     }
     double ps[1];
-    reduce_partials<1>(gl(S.part), c.G, ps);
+    finish_partials<1>(gl(S.part), c.G, pl, ps);
     const double rr = st->rr;
     const T alpha = (T)(rr / ps[0]);
     {
@@ -1851,6 +1894,8 @@ __device__ __forceinline__ void cg_direction_body(const Grp& g, int vb)
     constexpr int U = 4;
     int i = beg + threadIdx.x;
     // first batch loaded before the reduction (independent of beta)
+    PartialLoads<2> pl;  // partials first, then the first batch of rows
+    load_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, pl);
     T pi[U], ri[U], di[U];
     const bool first = i + (U - 1) * kBlock < end;
     if (first) {
@@ -1871,7 +1916,7 @@ __device__ __forceinline__ void cg_direction_body(const Grp& g, int vb)
         }
     }
     double s[2];
-    reduce_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, s);
+    finish_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, pl, s);
     const double rnorm = sqrt(s[0]);
     const double error = rnorm / st->bnorm;
     const double tol = gl(S.prm)->tol;
@@ -2039,64 +2084,61 @@ __device__ __forceinline__ void bi_dir_body(const Grp& g, int vb)
         if (c.b == 0 && threadIdx.x == 0) st->done = 1;
         return;
     }
-    double s[2];
-    reduce_partials<2>(gl(S.part), c.G, s);
-    const double tol = gl(S.prm)->tol;
+    // the partials' loads first, then the first batch of rows (p and v are
+    // read at the first iteration too, unused), then the reduction and tests
+    PartialLoads<2> pl;
+    load_partials<2>(gl(S.part), c.G, pl);
     const int it = st->iter;
-    if (it > 0) {
-        const double err = sqrt(s[0]) / st->bnorm;
-        if (err < tol || fabs(st->omega) < DBL_MIN) {
-            if (c.b == 0 && threadIdx.x == 0) {
-                st->error = err;
-                st->done = (err < tol) ? 1 : 3;
-            }
-            return;
-        }
-        if (it >= st->max_iter) {
-            if (c.b == 0 && threadIdx.x == 0) {
-                st->error = err;
-                st->done = 2;
-            }
-            return;
-        }
-    }
-    const double rho1 = s[1];
-    if (fabs(rho1) < DBL_MIN) {
-        if (c.b == 0 && threadIdx.x == 0) st->done = 6;
-        return;
-    }
     const GVec<T> v = gvec(kv_of<T>(S));
     T rr[kVecU], pp[kVecU], vv[kVecU], dd[kVecU];
-    if (it == 0) {
-        vec_chunk_batched(
-            S.n, c.G, c.b,
-            [&](int k, int u) {
-                rr[u] = v.r[k];
-                dd[u] = v.d[k];
-            },
-            [&](int k, int u) {
-                const T pi = rr[u];
-                v.p[k] = pi;
-                v.ph[k] = pi / dd[u];
-            });
-    } else {
-        const T beta = (T)((rho1 / st->rho0) * (st->alpha / st->omega));
-        const T omega = (T)st->omega;
-        vec_chunk_batched(
-            S.n, c.G, c.b,
-            [&](int k, int u) {
-                rr[u] = v.r[k];
-                pp[u] = v.p[k];
-                vv[u] = v.v[k];
-                dd[u] = v.d[k];
-            },
-            [&](int k, int u) {
-                const T pi = rr[u] + beta * (pp[u] - omega * vv[u]);
-                v.p[k] = pi;
-                v.ph[k] = pi / dd[u];
-            });
-    }
-    if (c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
+    double rho1 = 0.0;
+    T beta = 0, omega = 0;
+    const bool go = vec_chunk_batched_mid(
+        S.n, c.G, c.b,
+        [&](int k, int u) {
+            rr[u] = v.r[k];
+            pp[u] = v.p[k];
+            vv[u] = v.v[k];
+            dd[u] = v.d[k];
+        },
+        [&]() -> bool {
+            double s[2];
+            finish_partials<2>(gl(S.part), c.G, pl, s);
+            const double tol = gl(S.prm)->tol;
+            if (it > 0) {
+                const double err = sqrt(s[0]) / st->bnorm;
+                if (err < tol || fabs(st->omega) < DBL_MIN) {
+                    if (c.b == 0 && threadIdx.x == 0) {
+                        st->error = err;
+                        st->done = (err < tol) ? 1 : 3;
+                    }
+                    return false;
+                }
+                if (it >= st->max_iter) {
+                    if (c.b == 0 && threadIdx.x == 0) {
+                        st->error = err;
+                        st->done = 2;
+                    }
+                    return false;
+                }
+            }
+            rho1 = s[1];
+            if (fabs(rho1) < DBL_MIN) {
+                if (c.b == 0 && threadIdx.x == 0) st->done = 6;
+                return false;
+            }
+            if (it > 0) {
+                beta = (T)((rho1 / st->rho0) * (st->alpha / st->omega));
+                omega = (T)st->omega;
+            }
+            return true;
+        },
+        [&](int k, int u) {
+            const T pi = it == 0 ? rr[u] : rr[u] + beta * (pp[u] - omega * vv[u]);
+            v.p[k] = pi;
+            v.ph[k] = pi / dd[u];
+        });
+    if (go && c.b == 0 && threadIdx.x == 0) st->rho1 = rho1;
 }
 
 template <typename T>
@@ -2141,22 +2183,29 @@ __device__ __forceinline__ void bi_s_body(const Grp& g, int vb)
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
-    double sv[1];
-    reduce_partials<1>(gl(S.part) + 4 * (size_t)c.G, c.G, sv);
+    PartialLoads<1> pl;  // partials first, then the first batch of rows
+    load_partials<1>(gl(S.part) + 4 * (size_t)c.G, c.G, pl);
     const double rho1 = st->rho1;
-    const double alpha_d = rho1 / sv[0];
-    const T alpha = (T)alpha_d;
+    double alpha_d = 0.0;
+    T alpha = 0;
     const GVec<T> v = gvec(kv_of<T>(S));
     {
         const int b = c.b;
         double acc[1] = {0.0};
         T rr[kVecU], vv[kVecU], dd[kVecU];
-        vec_chunk_batched(
+        vec_chunk_batched_mid(
             S.n, c.G, b,
             [&](int k, int u) {
                 rr[u] = v.r[k];
                 vv[u] = v.v[k];
                 dd[u] = v.d[k];
+            },
+            [&]() -> bool {
+                double sv[1];
+                finish_partials<1>(gl(S.part) + 4 * (size_t)c.G, c.G, pl, sv);
+                alpha_d = rho1 / sv[0];
+                alpha = (T)alpha_d;
+                return true;
             },
             [&](int k, int u) {
                 const T si = rr[u] - alpha * vv[u];
@@ -2228,17 +2277,17 @@ __device__ __forceinline__ void bi_update_body(const Grp& g, int vb)
     const SysDev& S = *c.S;
     OGSX_GL KrylovState* st = gl(S.st);
     if (st->done || st->pend) return;
-    double sv[2];
-    reduce_partials<2>(gl(S.part) + 12 * (size_t)c.G, c.G, sv);
-    const double omega_d = (sv[0] > DBL_MIN) ? sv[1] / sv[0] : 1.0;
-    const T omega = (T)omega_d;
+    PartialLoads<2> pl;  // partials first, then the first batch of rows
+    load_partials<2>(gl(S.part) + 12 * (size_t)c.G, c.G, pl);
+    double omega_d = 0.0;
+    T omega = 0;
     const T alpha = (T)st->alpha;
     const GVec<T> v = gvec(kv_of<T>(S));
     {
         const int b = c.b;
         double acc[2] = {0.0, 0.0};
         T xx[kVecU], ph[kVecU], sh[kVecU], ss[kVecU], tt[kVecU], r0[kVecU];
-        vec_chunk_batched(
+        vec_chunk_batched_mid(
             S.n, c.G, b,
             [&](int k, int u) {
                 xx[u] = v.x[k];
@@ -2247,6 +2296,13 @@ __device__ __forceinline__ void bi_update_body(const Grp& g, int vb)
                 ss[u] = v.s[k];
                 tt[u] = v.t[k];
                 r0[u] = v.r0[k];
+            },
+            [&]() -> bool {
+                double sv[2];
+                finish_partials<2>(gl(S.part) + 12 * (size_t)c.G, c.G, pl, sv);
+                omega_d = (sv[0] > DBL_MIN) ? sv[1] / sv[0] : 1.0;
+                omega = (T)omega_d;
+                return true;
             },
             [&](int k, int u) {
                 v.x[k] = xx[u] + (alpha * ph[u] + omega * sh[u]);
