@@ -3039,13 +3039,14 @@ __global__ __launch_bounds__(kBlock) void k_bc_row(int m, const int* idx, const 
 // 32 lanes per row: the row's columns are read in one coalesced pass, and
 // the lifts are applied one Dirichlet column at a time in column order
 // (ballot over the lanes that hold one, then each lane's product in turn).
-__device__ __forceinline__ void bc_sym_rows(int n, const int* ptr, const int* col,
+__device__ __forceinline__ void bc_sym_rows(int n, const int* rows, const int* ptr, const int* col,
                                             const int* diagpos, const int8_t* flag, const double* g,
                                             double* A, double* b, int first, int step)
 {
     const int lane = threadIdx.x & 31;
     const int half = (threadIdx.x & 63) >> 5;
-    for (int i = first; i < n; i += step) {
+    for (int t = first; t < n; t += step) {
+        const int i = rows ? rows[t] : t;
         const int p0 = ptr[i], p1 = ptr[i + 1];
         if (flag[i]) {
             const int dp = diagpos[i];
@@ -3078,11 +3079,14 @@ __device__ __forceinline__ void bc_sym_rows(int n, const int* ptr, const int* co
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bc_sym(int n, const int* ptr, const int* col,
-                                                   const int* diagpos, const int8_t* flag,
-                                                   const double* g, double* A, double* b)
+// Symmetric elimination over the m rows of `rows` (rows the elimination can
+// change; the others are left as they are).
+__global__ __launch_bounds__(kBlock) void k_bc_sym(int m, const int* rows, const int* ptr,
+                                                   const int* col, const int* diagpos,
+                                                   const int8_t* flag, const double* g, double* A,
+                                                   double* b)
 {
-    bc_sym_rows(n, ptr, col, diagpos, flag, g, A, b,
+    bc_sym_rows(m, rows, ptr, col, diagpos, flag, g, A, b,
                 blockIdx.x * (kBlock / 32) + (int)(threadIdx.x >> 5), gridDim.x * (kBlock / 32));
 }
 
@@ -3104,11 +3108,12 @@ struct StepArgs {
     const double* st_val;
     const int32_t* bc_idx;   // stored Dirichlet list
     const double* bc_val;
+    const int32_t* bc_rows;  // rows the symmetric elimination can change
     int8_t* bcflag;
     double* bcg;
     double* uold;
     const double* x;
-    int n, nnz, ne, st_nu, bc_m, shift;
+    int n, nnz, ne, st_nu, bc_m, bc_rows_m, shift;
 };
 
 // Launch geometry of one phase: block b works for member blk2mem[b] as its
@@ -3204,12 +3209,12 @@ __global__ __launch_bounds__(kBlock) void k_step_bc_mark(StepMap m, int8_t on)
     bc_mark_one(k, S.bc_idx, S.bc_val, S.bcflag, S.bcg, on);
 }
 
-// symmetric elimination (as k_bc_sym) over the member's rows with its blocks
+// symmetric elimination (as k_bc_sym) over the member's listed rows with its blocks
 __global__ __launch_bounds__(kBlock) void k_step_bc_sym(StepMap m)
 {
     const StepCtx c = step_ctx(m);
     const StepArgs& S = *c.S;
-    bc_sym_rows(S.n, S.ptr, S.col, S.diagpos, S.bcflag, S.bcg, S.a.A, S.a.b,
+    bc_sym_rows(S.bc_rows_m, S.bc_rows, S.ptr, S.col, S.diagpos, S.bcflag, S.bcg, S.a.A, S.a.b,
                 c.lb * (kBlock / 32) + (int)(threadIdx.x >> 5), c.nb * (kBlock / 32));
 }
 
