@@ -1604,6 +1604,30 @@ __global__ __launch_bounds__(kBlock) void k_diag(Grp g)
 
 // CRS values -> SELL-64 values (padding 0) of one precision, in the SELL
 // layout (tri = 0) or the slice-triple layout (tri = 1).
+// Value copies into a slice layout: store(p, A[src[p]]) (0 where src[p] < 0)
+// for the block's chunk of the layout's positions, 8 positions per thread at
+// a time so that a thread's index loads, then its gathers of A, are each one
+// memory round trip.
+template <typename Store>
+__device__ __forceinline__ void copy_to_layout(const Ctx& c, int npos, const OGSX_GL int* src,
+                                               const OGSX_GL double* A, Store&& store)
+{
+    constexpr int U = 8;
+    int beg, end;
+    chunk_of(npos, c.G, c.b, beg, end);
+    for (int i = beg + threadIdx.x; i < end; i += U * kBlock) {
+        int k[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) k[u] = i + u * kBlock < end ? src[i + u * kBlock] : -1;
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = k[u] >= 0 ? A[k[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (i + u * kBlock < end) store(i + u * kBlock, v[u]);
+    }
+}
+
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g, int tri)
 {
@@ -1616,10 +1640,7 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_g(Grp g, int tri)
         sv = gl(S.sval32);
     const OGSX_GL int* src = gl(tri ? S.tri_src : S.sell_src);
     const OGSX_GL double* A = gl(S.A);
-    vec_vblocks(c, tri ? S.npad3 : S.npad, [&](int p) {
-        const int k = src[p];
-        sv[p] = k >= 0 ? (VT)A[k] : (VT)0;
-    });
+    copy_to_layout(c, tri ? S.npad3 : S.npad, src, A, [&](int p, double v) { sv[p] = (VT)v; });
 }
 
 // Both value copies of an IR system in one pass when they share a layout:
@@ -1632,9 +1653,7 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_both_g(Grp g, int tri)
     OGSX_GL float* sv32 = gl(S.sval32);
     const OGSX_GL int* src = gl(tri ? S.tri_src : S.sell_src);
     const OGSX_GL double* A = gl(S.A);
-    vec_vblocks(c, tri ? S.npad3 : S.npad, [&](int p) {
-        const int k = src[p];
-        const double v = k >= 0 ? A[k] : 0.0;
+    copy_to_layout(c, tri ? S.npad3 : S.npad, src, A, [&](int p, double v) {
         sv64[p] = v;
         sv32[p] = (float)v;
     });
