@@ -461,7 +461,7 @@ def heat_mode(size, steps, cpu=True):
     # algorithmic bytes of one SpMV: every nonzero value read once, x read
     # once, y written once
     width = 4 if best is res["fp32_ir"] else 8
-    alg = width * nnz + 2 * width * n
+    alg = _spmv_alg_bytes(width, nnz, n, best.get("spmv_format"))
     cpu_base = None
     if cpu:
         from oracle import ffi
@@ -698,11 +698,12 @@ def main():
     best_name = "fp32_ir" if best is ir else "fp64"
     sp = best["spmv"]
     peak = HBM_PEAK_GBS
-    # algorithmic bytes of one SpMV: every nonzero value read once, x read
+    # algorithmic bytes of one SpMV: every value the format must read once
+    # (half the off-diagonal ones for the symmetric diagonal copy), x read
     # once, y written once (no index, padding or stored zero is counted; the
-    # tuned format's own stored bytes ride along as stored_bytes_per_launch)
+    # format's own stored bytes ride along as stored_bytes_per_launch)
     width = 4 if best is ir else 8
-    alg = width * _nnz(sc) + 2 * width * sc.n
+    alg = _spmv_alg_bytes(width, _nnz(sc), sc.n, best["spmv_format"])
     achieved = alg / sp["avg_ms"] / 1e6
     in_ms = best["spmv_in_solve"]["avg_ms"]
     roof = dict(bound="hbm", achieved=achieved, peak=peak, unit="GB/s",
@@ -780,6 +781,14 @@ def main():
 def _nnz(sc):
     n1 = sc.mesh.shape[0] + 1
     return (3 * n1 - 2) ** 3
+
+
+def _spmv_alg_bytes(width, nnz, n, fmt):
+    """Algorithmic bytes of one SpMV: every stored value the format must read
+    once (all nnz; for the symmetric diagonal copy, format 25, the diagonal
+    and one triangle: (nnz + n) / 2), x read once, y written once."""
+    vals = (nnz + n) // 2 if fmt == 25 else nnz
+    return width * vals + 2 * width * n
 
 
 if __name__ == "__main__":

@@ -389,7 +389,15 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * partial sums: a different rounding order).  With 1, 2 and 4-64 the fused
  * dot products of a solve accumulate rows in another order, so solves
  * round differently (within the usual FP64 tolerance).  The CRS arrays stay the system's matrix;
- * SELL copies are refreshed after A changes. */
+ * SELL copies are refreshed after A changes.  The FP32 matrix of an FP32-IR
+ * system that is symmetric by construction (symmetric tensors, symmetric
+ * Dirichlet elimination, no advection) on a banded pattern (at most 16
+ * positive column offsets) is instead held as format 25: its diagonal and
+ * upper diagonals, each lower entry read from its mirror -- half the values.
+ * It is chosen by structure, never by timing (so results do not depend on
+ * the tuner), is not selectable here, and sums rows in another order than
+ * multiVec, which the FP64 refinement absorbs; a format forced here keeps
+ * the FP32 matrix off it. */
 int ogsx_set_spmv_variant(ogsx_system* s, int variant);
 int ogsx_get_spmv_variant(ogsx_system* s, int* variant);
 /* The SpMV formats in use for the FP64 and (IR systems, else 0) FP32

@@ -100,6 +100,11 @@ struct PatternLayout {
     std::vector<uint16_t> tid;
     int64_t npad3 = 0;
     bool tri_ok = false;
+    // symmetric diagonal storage (format 25): the positive column offsets of
+    // the pattern, ascending, when there are at most kDiaMax of them
+    std::vector<int> doff;
+    std::vector<int> dsrc;  // CRS position of every stored entry, -1 for a zero
+    bool dia_ok = false;
 };
 
 struct ogsx_pattern {
@@ -175,6 +180,13 @@ struct ogsx_system {
     int64_t npad3 = 0;
     int *tri_ptr = nullptr, *tri_cnt = nullptr, *tri_off = nullptr, *tri_src = nullptr;
     uint16_t* tri_id = nullptr;
+    bool dia_ok = false;  // symmetric diagonal FP32 copy available (format 25)
+    int* dia_off = nullptr;
+    int* dia_src = nullptr;  // CRS position of every stored entry (-1: zero)
+    float* dia_val32 = nullptr;
+    int dia_k = 0;
+    uint64_t dia32_version = 0;
+    int fmt32_tuned = 0;  // the tuned FP32 format while format 25 stands in for it
     // values in SELL-64 (layout 0) or slice-triple (layout 1) order, as the
     // precision's format reads them
     double* sval64 = nullptr;
@@ -337,11 +349,14 @@ MatView<VT> mat_view(const ogsx_system* s, const VT* val)
         sv = s->sval64;
     else
         sv = s->sval32;
+    const VT* dval = nullptr;
+    if constexpr (sizeof(VT) == 4) dval = s->dia_val32;
     return MatView<VT>{s->n,        s->n_tiles,  s->ptr,      s->col,     s->col16,
                        val,         s->tile_row, s->tile_cb,  s->n_slices, s->sell_ptr,
                        s->sell_w,   s->sell_cb,  s->sell_c16, s->sell_c32, sv,
                        s->sell_pid, s->pat_delta, s->pat_w,   s->pat_cnt,  s->tri_ptr,
-                       s->tri_id,   s->tri_cnt,  s->tri_off};
+                       s->tri_id,   s->tri_cnt,  s->tri_off,  dval ? s->dia_off : nullptr,
+                       dval,        dval ? s->dia_k : 0};
 }
 
 inline bool is_sell(int f)
@@ -605,6 +620,44 @@ void build_triples(const PatternHost& h, PatternLayout& L)
     L.tri_ok = true;
 }
 
+// Positive column offsets of the pattern (format 25), when every row's
+// entries right of the diagonal sit at one of at most kDiaMax offsets.
+void build_dia(const PatternHost& h, PatternLayout& L)
+{
+    std::vector<int> offs, row, prev;
+    for (int64_t i = 0; i < h.n; i++) {
+        row.clear();
+        for (int64_t k = h.ptr[i]; k < h.ptr[i + 1]; k++)
+            if (h.col[k] > i) row.push_back((int)(h.col[k] - i));
+        if (row == prev) continue;  // most rows repeat their neighbour's offsets
+        prev = row;
+        for (int o : row)
+            if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+                if ((int)offs.size() == kDiaMax) return;  // too many: not built
+                offs.push_back(o);
+            }
+    }
+    std::sort(offs.begin(), offs.end());
+    if (offs.empty()) return;
+    // position map: entry (k + 1) n + i holds (i, i + offs[k]), entry i the
+    // diagonal (rows and offsets both ascending: one merge per row)
+    const int K = (int)offs.size();
+    const int64_t n = h.n;
+    L.dsrc.assign((size_t)(K + 1) * (size_t)n, -1);
+    for (int64_t i = 0; i < n; i++) {
+        int k = 0;
+        for (int64_t p = h.ptr[i]; p < h.ptr[i + 1]; p++) {
+            const int64_t o = h.col[p] - i;
+            if (o == 0) L.dsrc[i] = (int)p;
+            if (o <= 0) continue;
+            while (k < K && offs[k] < o) k++;
+            if (k < K && offs[k] == o) L.dsrc[(size_t)(k + 1) * n + i] = (int)p;
+        }
+    }
+    L.doff = offs;
+    L.dia_ok = true;
+}
+
 void build_layout(const PatternHost& h, PatternLayout& L)
 {
     L.ptr32.resize(h.n + 1);
@@ -619,6 +672,7 @@ void build_layout(const PatternHost& h, PatternLayout& L)
     L.npad = L.sptr.back();
     build_pattern_dict(h, L.sw, L);
     build_triples(h, L);
+    build_dia(h, L);
 }
 
 // Device bytes a system on pattern p allocates: ogsx_system_create plus
@@ -649,6 +703,8 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
              b((int64_t)L.toff.size(), 4) + b(L.npad3, 4) + extra * 8 +
              (precision == OGSX_FP32_IR ? extra * 4 : 0);
     }
+    if (L.dia_ok && precision == OGSX_FP32_IR)  // offsets, position map, values
+        t += b((int64_t)L.doff.size(), 4) + 2 * b(((int64_t)L.doff.size() + 1) * n, 4);
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
          b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
@@ -723,6 +779,7 @@ int prof_flush(ogsx_system* s)
         case kFmtTri5: { constexpr int L = kFmtTri5; CALL; } break;           \
         case kFmtTri4C: { constexpr int L = kFmtTri4C; CALL; } break;         \
         case kFmtCrs1: { constexpr int L = kFmtCrs1; CALL; } break;           \
+        case kFmtSymDia: { constexpr int L = kFmtSymDia; CALL; } break;       \
         case 4: { constexpr int L = 4; CALL; } break;           \
         case 8: { constexpr int L = 8; CALL; } break;           \
         case 16: { constexpr int L = 16; CALL; } break;         \
@@ -814,6 +871,8 @@ double spmv_bytes(const ogsx_system* s, int fmt, int sval, int sx)
                       fmt == kFmtSell16U7)
                          ? 2
                          : 4;
+    if (fmt == kFmtSymDia)  // diagonal and the upper diagonals (zeros included)
+        return (double)(s->dia_k + 1) * n * sval + 2 * n * sx;
     if (is_tri(fmt))  // values incl. zero entries, slice table (first entry, stencil id)
         return (double)s->npad3 * sval + 6.0 * s->n_slices + 2 * n * sx;
     if (is_pat(fmt))  // values, slice table, one id per row
@@ -1093,6 +1152,17 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
                 return cleanup(fail(OGSX_EHIP, "upload of the slice triples failed"));
             s->tri_ok = true;
         }
+        if (Ly.dia_ok && precision == OGSX_FP32_IR) {
+            s->dia_k = (int)Ly.doff.size();
+            if ((rc = dalloc(s, &s->dia_off, Ly.doff.size())) ||
+                (rc = dalloc(s, &s->dia_src, Ly.dsrc.size())) ||
+                (rc = dalloc(s, &s->dia_val32, ((size_t)s->dia_k + 1) * (size_t)h.n)))
+                return cleanup(rc);
+            if (up(s->dia_off, Ly.doff.data(), 4 * Ly.doff.size()) ||
+                up(s->dia_src, Ly.dsrc.data(), 4 * Ly.dsrc.size()))
+                return cleanup(fail(OGSX_EHIP, "upload of the diagonal layout failed"));
+            s->dia_ok = true;
+        }
     }
     // the 32-bit-column SELL and tiled layouts are tuner candidates only when
     // 16-bit slice offsets do not fit; otherwise they are built on request
@@ -1112,6 +1182,7 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
         d.diagpos = s->diagpos;
         d.sell_src = s->sell_src;
         d.tri_src = s->tri_src;
+        d.dia_src = s->dia_src;
         d.r64 = s->r64;
         d.part = s->part;
         d.red = s->red;

@@ -275,6 +275,13 @@ struct MatView {
     const uint16_t* tri_id;
     const int* tri_cnt;
     const int* tri_off;
+    // symmetric diagonal storage (FP32 inner solves of symmetric systems on
+    // banded patterns): dia_val[0 .. n) the diagonal, dia_val[(k + 1) n + i]
+    // the entry (i, i + dia_off[k]) for the dia_k positive offsets (0 where
+    // the pattern has no such entry); entry (i, i - o) is the stored (i - o, i)
+    const int* dia_off;
+    const VT* dia_val;
+    int dia_k;
 };
 
 // MatView with global-address-space pointers, as the device loops use it.
@@ -303,6 +310,9 @@ struct GMat {
     const OGSX_GL uint16_t* tri_id;
     const OGSX_GL int* tri_cnt;
     const OGSX_GL int* tri_off;
+    const OGSX_GL int* dia_off;
+    const OGSX_GL VT* dia_val;
+    int dia_k;
 };
 
 template <typename VT>
@@ -313,7 +323,8 @@ __device__ __forceinline__ GMat<VT> gmat(const MatView<VT>& m)
                     m.n_slices,   gl(m.sell_ptr),  gl(m.sell_w),   gl(m.sell_cb),
                     gl(m.sell_c16), gl(m.sell_c32), gl(m.sval),    gl(m.sell_pid),
                     gl(m.pat_delta), m.pat_w,       m.pat_cnt,     gl(m.tri_ptr),
-                    gl(m.tri_id),   gl(m.tri_cnt),  gl(m.tri_off)};
+                    gl(m.tri_id),   gl(m.tri_cnt),  gl(m.tri_off), gl(m.dia_off),
+                    gl(m.dia_val),  m.dia_k};
 }
 
 constexpr int kTileU = 8;                 // nonzeros per thread per tile
@@ -340,6 +351,9 @@ constexpr int kPatLds = 4096;             // dictionary entries staged in LDS
 constexpr int kFmtTri5 = 19;              // SELL-64 slice triples, 5 in flight, NT
 constexpr int kFmtTri4C = 20;             // ... 4 in flight, cached loads
 constexpr int kTriMax = 16;               // triples per slice stencil
+constexpr int kFmtSymDia = 25;            // symmetric diagonal storage, FP32 inner
+                                          // solves only (chosen by structure, not timed)
+constexpr int kDiaMax = 16;               // positive offsets it takes at most
 constexpr int kFmtCrs1 = 9;               // plain CRS, one thread per row (needs only
                                           // ptr, col, val)
 
@@ -761,6 +775,47 @@ __device__ __forceinline__ void spmv_crs1(const GMat<VT>& m, const OGSX_GL XT* _
 
 // Format dispatch: FMT = 4..64 lanes per row (vector CRS), or a tiled,
 // SELL-64 or plain CRS format.
+// Symmetric diagonal storage: y_i = d_i x_i + sum_k (u_k[i] x[i + o_k] +
+// u_k[i - o_k] x[i - o_k]), one thread per row, offsets in ascending order
+// and for each the upper, then the lower entry.  Half the matrix values of
+// the full formats cross HBM; the lower entries are the upper ones of rows
+// o_k back, read again from cache (the blocks of one XCD take consecutive
+// chunks).  Not the reference's summation order, hence only for the FP32
+// matrix inside the FP64 refinement, whose FP64 residual stays exact.
+template <typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_symdia(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                            int G, int b, Epi&& epi)
+{
+    int beg, end;
+    chunk_of(m.n, G, b, beg, end);
+    const int n = m.n, K = m.dia_k;
+    const OGSX_GL VT* dv = m.dia_val;
+    // two rows per thread at a time (i and i + kBlock; the second clamped to
+    // the chunk's last row and not stored past it), sums before either
+    // epilogue, so the loads of both rows are in flight together
+    for (int i = beg + threadIdx.x; i < end; i += 2 * kBlock) {
+        int r[2] = {i, min(i + kBlock, end - 1)};
+        ACC s[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) s[q] = (ACC)dv[r[q]] * (ACC)x[r[q]];
+#pragma unroll 4
+        for (int k = 0; k < K; k++) {
+            const int o = m.dia_off[k];
+            const OGSX_GL VT* u = dv + (int64_t)(k + 1) * n;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int up = min(r[q] + o, n - 1), lo = r[q] - o;
+                const ACC a1 = (ACC)u[r[q]], x1 = (ACC)x[up];  // u[i] = 0 where i + o >= n
+                const ACC a2 = lo >= 0 ? (ACC)u[max(lo, 0)] : (ACC)0, x2 = (ACC)x[max(lo, 0)];
+                s[q] = s[q] + a1 * x1;
+                s[q] = s[q] + a2 * x2;
+            }
+        }
+        epi(r[0], s[0]);
+        if (i + kBlock < end) epi(r[1], s[1]);
+    }
+}
+
 template <int FMT, typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
                                           int G, int b, Epi&& epi)
@@ -801,6 +856,8 @@ __device__ __forceinline__ void spmv_rows(const GMat<VT>& m, const OGSX_GL XT* _
         spmv_tri<4, false, VT, XT, ACC>(m, x, G, b, epi);
     else if constexpr (FMT == kFmtCrs1)
         spmv_crs1<VT, XT, ACC>(m, x, G, b, epi);
+    else if constexpr (FMT == kFmtSymDia)
+        spmv_symdia<VT, XT, ACC>(m, x, G, b, epi);
     else
         spmv_chunk<FMT, VT, XT, ACC>(m.n, m.ptr, m.col, m.val, x, G, b, epi);
 }
@@ -1521,6 +1578,7 @@ struct SysDev {
     const int* diagpos;
     const int* sell_src;
     const int* tri_src;  // CRS position of every slice-triple entry (-1 = zero)
+    const int* dia_src;  // ... of every symmetric-diagonal entry (-1 = zero)
     double* r64;         // IR residual
     double* part;        // partial sums [16][G]
     double* red;         // reduced scalars
@@ -1657,6 +1715,17 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_both_g(Grp g, int tri)
         sv64[p] = v;
         sv32[p] = (float)v;
     });
+}
+
+// FP32 symmetric diagonal copy of A (see spmv_symdia) through its position
+// map, as the slice-layout copies.
+__global__ __launch_bounds__(kBlock) void k_to_dia32_g(Grp g)
+{
+    const Ctx c = ctx_of(g);
+    const SysDev& S = *c.S;
+    OGSX_GL float* dv = (OGSX_GL float*)gl(S.m32.dia_val);
+    copy_to_layout(c, (S.m32.dia_k + 1) * S.n, gl(S.dia_src), gl(S.A),
+                   [&](int p, double v) { dv[p] = (float)v; });
 }
 
 __global__ __launch_bounds__(kBlock) void k_cast_f32_g(Grp g)

@@ -52,11 +52,18 @@ def test_group_shares_formats_within_slack(prec):
     """Systems starting together take a shared SpMV format when their own
     tuning timed it within 10 % or 4 us of their fastest (the larger); a
     forced format stays.
-    Results are those of each system alone (every format sums rows in CRS
-    order)."""
+    Results are those of each system alone in the same configuration
+    (every tuned format sums rows in CRS order)."""
     systems = [W.make_system(sc, prec) for sc in MIX]
     systems[1][1].set_spmv_variant(5)  # forced: must stay
-    alone = [W.run_gpu(sc, precision=prec, keep_steps=True) for sc in MIX]
+    # alone with the same configuration (a forced format also keeps an IR
+    # system's FP32 matrix off the symmetric diagonal copy, which rounds
+    # differently)
+    forced = W.make_system(MIX[1], prec)
+    forced[1].set_spmv_variant(5)
+    alone = [W.run_gpu(sc, precision=prec, keep_steps=True, system=forced if k == 1 else None)
+             for k, sc in enumerate(MIX)]
+    forced[1].close()
     mix = W.run_group(MIX, precision=prec, keep_steps=True, systems=systems)
     for k, a in enumerate(alone):
         _same(a, dict(info=mix["info"][k], steps=mix["steps"][k]))
@@ -71,6 +78,10 @@ def test_group_shares_formats_within_slack(prec):
             used.add((fp32, f))
             if k == 1:
                 assert f == 5
+                continue
+            if fp32 and f == 25:
+                # the symmetric diagonal copy is chosen by structure (a
+                # symmetric system on a banded pattern), not by the tuner
                 continue
             best = min(ms for _, ms in part)
             ms_f = dict(part)[f]

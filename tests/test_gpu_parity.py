@@ -271,6 +271,32 @@ def test_solve_with_slice_triples(prec, fmt):
     s.close()
 
 
+def test_symmetric_diagonal_fp32_copy():
+    """An FP32-IR system whose matrix is symmetric by construction, on a
+    banded pattern, runs its inner FP32 solve on the symmetric diagonal copy
+    (format 25: the diagonal and the upper diagonals, each lower entry read
+    from its mirror); the refinement still reaches the oracle's solutions.
+    With row-only Dirichlet rows (a nonsymmetric matrix) the FP32 matrix
+    keeps a tuned full format."""
+    for name in ("quad", "hex"):
+        sc = SCEN[name]
+        ref = _tight_reference(sc)
+        pat, s = W.make_system(sc, ox.FP32_IR)
+        res = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, precision=ox.FP32_IR,
+                        tol=1e-13, keep_steps=True, system=(pat, s))
+        assert s.spmv_format(True) == 25, name
+        for x, xr in zip(res["steps"], ref["steps"]):
+            assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < SOL_TOL
+        s.close()
+    sc = SCEN["quad"]
+    pat, s = W.make_system(sc, ox.FP32_IR)
+    res = W.run_gpu(sc, method=ox.BICGSTAB, dirichlet=ox.DIRICHLET_ROW, precision=ox.FP32_IR,
+                    system=(pat, s))
+    assert all(i["converged"] == 1 for i in res["info"])
+    assert s.spmv_format(True) != 25
+    s.close()
+
+
 @pytest.mark.parametrize("name,method,bc", [("quad", ox.CG, ox.DIRICHLET_SYMMETRIC),
                                             ("quad", ox.BICGSTAB, ox.DIRICHLET_ROW),
                                             ("hex", ox.CG, ox.DIRICHLET_SYMMETRIC),
