@@ -785,18 +785,21 @@ __device__ __forceinline__ void spmv_crs1(const GMat<VT>& m, const OGSX_GL XT* _
 #ifndef OGSX_DIA_ROWS
 #define OGSX_DIA_ROWS 2
 #endif
-template <typename VT, typename XT, typename ACC, typename Epi>
-__device__ __forceinline__ void spmv_symdia(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
-                                            int G, int b, Epi&& epi)
+// KC > 0: the offset count known at compile time (the loop over them fully
+// unrolled); KC = 0: m.dia_k at run time.
+template <int KC, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_symdia_k(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                              int G, int b, Epi&& epi)
 {
     int beg, end;
     chunk_of(m.n, G, b, beg, end);
-    const int n = m.n, K = m.dia_k;
+    const int n = m.n, K = KC > 0 ? KC : m.dia_k;
     const OGSX_GL VT* dv = m.dia_val;
     // R rows per thread at a time (i, i + kBlock, ...; those past the chunk
     // clamped to its last row and not stored), sums before any epilogue, so
     // the loads of the R rows are in flight together (128^3 FP32: R = 1, 2,
-    // 4 gave 36.1, 35.5, 43.8 us)
+    // 4 gave 36.1, 35.5, 43.8 us before the offset loop was unrolled; unrolled
+    // R = 1, 2: 31.5, 31.4 us)
     constexpr int R = OGSX_DIA_ROWS;
     for (int i = beg + threadIdx.x; i < end; i += R * kBlock) {
         int r[R];
@@ -805,7 +808,8 @@ __device__ __forceinline__ void spmv_symdia(const GMat<VT>& m, const OGSX_GL XT*
         ACC s[R];
 #pragma unroll
         for (int q = 0; q < R; q++) s[q] = (ACC)dv[r[q]] * (ACC)x[r[q]];
-#pragma unroll 4
+        constexpr int kUnroll = KC > 0 ? KC : 4;
+#pragma unroll kUnroll
         for (int k = 0; k < K; k++) {
             const int o = m.dia_off[k];
             const OGSX_GL VT* u = dv + (int64_t)(k + 1) * n;
@@ -822,6 +826,20 @@ __device__ __forceinline__ void spmv_symdia(const GMat<VT>& m, const OGSX_GL XT*
 #pragma unroll
         for (int q = 1; q < R; q++)
             if (i + q * kBlock < end) epi(r[q], s[q]);
+    }
+}
+
+// The offset counts of the structured meshes (4: 9-point quads, 7: 15-point
+// tets, 13: 27-point hexes) get fully unrolled instances.
+template <typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_symdia(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                            int G, int b, Epi&& epi)
+{
+    switch (m.dia_k) {
+        case 4: spmv_symdia_k<4, VT, XT, ACC>(m, x, G, b, epi); break;
+        case 7: spmv_symdia_k<7, VT, XT, ACC>(m, x, G, b, epi); break;
+        case 13: spmv_symdia_k<13, VT, XT, ACC>(m, x, G, b, epi); break;
+        default: spmv_symdia_k<0, VT, XT, ACC>(m, x, G, b, epi); break;
     }
 }
 
