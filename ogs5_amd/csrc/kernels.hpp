@@ -1921,6 +1921,11 @@ __device__ __forceinline__ void cg_update_body(const Grp& g, int vb)
     load_partials<1>(gl(S.part), c.G, pl);
     T xi[U], pi[U], ri[U], si[U], di[U];
     const bool first = i + (U - 1) * kBlock < end;
+    // a chunk of one full batch plus fewer than kBlock rows (1024-1280
+    // rows: 2048 chunks of a 2.1-2.6 M-row system) loads its extra row per
+    // thread with the batch, so the chunk costs one memory round trip
+    const bool tail = !SMALL && first && end - beg <= (U + 1) * kBlock && i + U * kBlock < end;
+    T xt = 0, pt = 0, rt = 0, st1 = 0, dt = 1;
     if (first) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -1930,6 +1935,14 @@ __device__ __forceinline__ void cg_update_body(const Grp& g, int vb)
             ri[u] = ld<VNT>(r + k);
             si[u] = ld<VNT>(sv + k);
             di[u] = ld<VNT>(d + k);
+        }
+        if (tail) {
+            const int k = i + U * kBlock;
+            xt = ld<VNT>(x + k);
+            pt = ld<VNT>(p + k);
+            rt = ld<VNT>(r + k);
+            st1 = ld<VNT>(sv + k);
+            dt = ld<VNT>(d + k);
         }
     } else if (SMALL && i < end) {
 #pragma unroll
@@ -1986,6 +1999,15 @@ __device__ __forceinline__ void cg_update_body(const Grp& g, int vb)
                 acc[1] += (double)zi * (double)rn;
             }
         }
+        if (tail) {  // the batch loop ran once: i is the thread's extra row
+            store_hint<VNT>(x + i, (T)(xt + alpha * pt));
+            const T rn = rt - alpha * st1;
+            store_hint<VNT>(r + i, rn);
+            const T zi = rn / dt;
+            acc[0] += (double)rn * (double)rn;
+            acc[1] += (double)zi * (double)rn;
+            i = end;
+        }
         for (; i < end; i += kBlock) {
             x[i] += alpha * p[i];
             const T rn = r[i] - alpha * sv[i];
@@ -2032,6 +2054,9 @@ __device__ __forceinline__ void cg_direction_body(const Grp& g, int vb)
     load_partials<2>(gl(S.part) + 4 * (size_t)c.G, c.G, pl);
     T pi[U], ri[U], di[U];
     const bool first = i + (U - 1) * kBlock < end;
+    // the extra row of a chunk of 1024-1280 rows, as in the update
+    const bool tail = !SMALL && first && end - beg <= (U + 1) * kBlock && i + U * kBlock < end;
+    T pt = 0, rt = 0, dt = 1;
     if (first) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -2039,6 +2064,12 @@ __device__ __forceinline__ void cg_direction_body(const Grp& g, int vb)
             pi[u] = ld<VNT>(p + k);
             ri[u] = ld<VNT>(r + k);
             di[u] = ld<VNT>(d + k);
+        }
+        if (tail) {
+            const int k = i + U * kBlock;
+            pt = ld<VNT>(p + k);
+            rt = ld<VNT>(r + k);
+            dt = ld<VNT>(d + k);
         }
     } else if (SMALL && i < end) {  // a chunk's one, partial batch (as in the update)
 #pragma unroll
@@ -2082,6 +2113,10 @@ __device__ __forceinline__ void cg_direction_body(const Grp& g, int vb)
             }
 #pragma unroll
             for (int u = 0; u < U; u++) p[i + u * kBlock] = ri[u] / di[u] + beta * pi[u];
+        }
+        if (tail) {  // the batch loop ran once: i is the thread's extra row
+            p[i] = rt / dt + beta * pt;
+            i = end;
         }
         for (; i < end; i += kBlock) p[i] = r[i] / d[i] + beta * p[i];
     }
