@@ -397,7 +397,10 @@ int ogsx_profile_read(ogsx_system* s, int kind, double* total_ms,
  * It is chosen by structure, never by timing (so results do not depend on
  * the tuner), is not selectable here, and sums rows in another order than
  * multiVec, which the FP64 refinement absorbs; a format forced here keeps
- * the FP32 matrix off it. */
+ * the FP32 matrix off it.  The FP64 matrix of such an FP64 system is held
+ * the same way (format 25 for fmt64), summed in multiVec's CRS order with
+ * the rows on the threads of the SELL-64 formats, so its solves equal those
+ * of every other tuned format bit for bit. */
 int ogsx_set_spmv_variant(ogsx_system* s, int variant);
 int ogsx_get_spmv_variant(ogsx_system* s, int* variant);
 /* The SpMV formats in use for the FP64 and (IR systems, else 0) FP32

@@ -341,7 +341,9 @@ class System:
 
     def spmv_format(self, fp32=False) -> int:
         """SpMV format in use for the FP64 (or, for IR systems, FP32) matrix:
-        the tuner's fastest, a forced variant, or one shared in a group."""
+        the tuner's fastest, a forced variant, one shared in a group, or 25
+        (the symmetric diagonal copy of a symmetric system on a banded
+        pattern, chosen by structure at the solve)."""
         f64, f32 = C.c_int(), C.c_int()
         check(lib().ogsx_get_spmv_formats(self.h, C.byref(f64), C.byref(f32)),
               "ogsx_get_spmv_formats")

@@ -183,9 +183,11 @@ struct ogsx_system {
     bool dia_ok = false;  // symmetric diagonal FP32 copy available (format 25)
     int* dia_off = nullptr;
     int* dia_src = nullptr;  // CRS position of every stored entry (-1: zero)
-    float* dia_val32 = nullptr;
+    float* dia_val32 = nullptr;   // FP32 copy (FP32-IR systems)
+    double* dia_val64 = nullptr;  // FP64 copy (FP64 systems)
     int dia_k = 0;
-    uint64_t dia32_version = 0;
+    uint64_t dia32_version = 0, dia64_version = 0;
+    int fmt64_tuned = 0;  // the tuner's FP64 format while format 25 stands in
     int fmt32_tuned = 0;  // the tuned FP32 format while format 25 stands in for it
     // values in SELL-64 (layout 0) or slice-triple (layout 1) order, as the
     // precision's format reads them
@@ -350,7 +352,10 @@ MatView<VT> mat_view(const ogsx_system* s, const VT* val)
     else
         sv = s->sval32;
     const VT* dval = nullptr;
-    if constexpr (sizeof(VT) == 4) dval = s->dia_val32;
+    if constexpr (sizeof(VT) == 4)
+        dval = s->dia_val32;
+    else
+        dval = s->dia_val64;
     return MatView<VT>{s->n,        s->n_tiles,  s->ptr,      s->col,     s->col16,
                        val,         s->tile_row, s->tile_cb,  s->n_slices, s->sell_ptr,
                        s->sell_w,   s->sell_cb,  s->sell_c16, s->sell_c32, sv,
@@ -469,6 +474,18 @@ void build_pattern_dict(const PatternHost& h, const std::vector<int>& sw, Patter
 
 // Brings the SELL value copies up to date with A (after assembly / BC), in
 // the layout of the precision's format (SELL-64 or slice triples).
+// Refreshes the FP64 symmetric diagonal copy from A when the FP64 format is
+// the diagonal one and A changed since the last copy.
+int sync_dia64(ogsx_system* s, hipStream_t st)
+{
+    if (s->fmt64 != kFmtSymDia || s->dia64_version == s->a_version) return OGSX_OK;
+    if (!s->dia_val64) return fail(OGSX_ESTATE, "no FP64 diagonal copy on this system");
+    k_to_dia_g<double><<<s->G, kBlock, 0, st>>>(Grp{s->dev, nullptr, nullptr});
+    HIP_TRY(hipGetLastError());
+    s->dia64_version = s->a_version;
+    return OGSX_OK;
+}
+
 int sync_sell(ogsx_system* s, bool want64, bool want32)
 {
     const Grp g1{s->dev, nullptr, nullptr};
@@ -703,8 +720,9 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
              b((int64_t)L.toff.size(), 4) + b(L.npad3, 4) + extra * 8 +
              (precision == OGSX_FP32_IR ? extra * 4 : 0);
     }
-    if (L.dia_ok && precision == OGSX_FP32_IR)  // offsets, position map, values
-        t += b((int64_t)L.doff.size(), 4) + 2 * b(((int64_t)L.doff.size() + 1) * n, 4);
+    if (L.dia_ok)  // offsets, position map, values (FP32 or FP64)
+        t += b((int64_t)L.doff.size(), 4) + b(((int64_t)L.doff.size() + 1) * n, 4) +
+             b(((int64_t)L.doff.size() + 1) * n, precision == OGSX_FP32_IR ? 4 : 8);
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
          b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
@@ -1152,11 +1170,13 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
                 return cleanup(fail(OGSX_EHIP, "upload of the slice triples failed"));
             s->tri_ok = true;
         }
-        if (Ly.dia_ok && precision == OGSX_FP32_IR) {
+        if (Ly.dia_ok) {
             s->dia_k = (int)Ly.doff.size();
+            const size_t nd = ((size_t)s->dia_k + 1) * (size_t)h.n;
             if ((rc = dalloc(s, &s->dia_off, Ly.doff.size())) ||
                 (rc = dalloc(s, &s->dia_src, Ly.dsrc.size())) ||
-                (rc = dalloc(s, &s->dia_val32, ((size_t)s->dia_k + 1) * (size_t)h.n)))
+                (rc = precision == OGSX_FP32_IR ? dalloc(s, &s->dia_val32, nd)
+                                                : dalloc(s, &s->dia_val64, nd)))
                 return cleanup(rc);
             if (up(s->dia_off, Ly.doff.data(), 4 * Ly.doff.size()) ||
                 up(s->dia_src, Ly.dsrc.data(), 4 * Ly.dsrc.size()))
@@ -2284,6 +2304,7 @@ int ogsx_spmv(ogsx_system* s, const double* x, double* y)
     int pk = prof_begin(s, OGSX_PROF_SPMV64);
     OK_TRY(autotune(s));
     OK_TRY(sync_sell(s, is_sell(s->fmt64), false));
+    OK_TRY(sync_dia64(s, s->stream));
     FMT_DISPATCH(s->fmt64, (k_spmv<L, double, double><<<s->G, kBlock, 0, s->stream>>>(
                                mat_view(s, (const double*)s->A), s->v64.r, s->v64.s)));
     prof_end(s, pk);
@@ -2418,6 +2439,7 @@ int ogsx_bench_spmv(ogsx_system* s, int fp32, int reps, double* avg_ms, double* 
     // the solve's own SpMV kernel (k_cg_spmv), on p = 1
     if (!fp32) {
         OK_TRY(sync_sell(s, is_sell(s->fmt64), false));
+        OK_TRY(sync_dia64(s, s->stream));
         k_fill<double><<<blocks_for(s->n, kBlock * 4), kBlock, 0, s->stream>>>(s->n, s->v64.p, 1.0);
         k_set_done<<<1, 64, 0, s->stream>>>(s->st, 0);
         (void)hipEventRecord(e0, s->stream);

@@ -780,8 +780,9 @@ __device__ __forceinline__ void spmv_crs1(const GMat<VT>& m, const OGSX_GL XT* _
 // and for each the upper, then the lower entry.  Half the matrix values of
 // the full formats cross HBM; the lower entries are the upper ones of rows
 // o_k back, read again from cache (the blocks of one XCD take consecutive
-// chunks).  Not the reference's summation order, hence only for the FP32
-// matrix inside the FP64 refinement, whose FP64 residual stays exact.
+// chunks).  Not the reference's summation order: this form serves the FP32
+// matrix inside the FP64 refinement, whose FP64 residual stays exact; the
+// FP64 matrix uses spmv_symdia_crs_k below.
 #ifndef OGSX_DIA_ROWS
 #define OGSX_DIA_ROWS 2
 #endif
@@ -829,12 +830,90 @@ __device__ __forceinline__ void spmv_symdia_k(const GMat<VT>& m, const OGSX_GL X
     }
 }
 
+// The same storage summed in the CRS row's column order: the lower entries
+// from the largest offset down, the diagonal, the upper entries from the
+// smallest offset up, no fused multiply-adds.  A stored zero (a column the
+// row does not have) adds +-0 to the running sum, which leaves it unchanged
+// unless it is itself zero, so the result equals the CRS kernels' bit for
+// bit (up to the sign of an all-zero row): the FP64 matrix of a symmetric
+// system reads half its values.
+template <int KC, typename VT, typename XT, typename ACC, typename Epi>
+__device__ __forceinline__ void spmv_symdia_crs_k(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
+                                                  int G, int b, Epi&& epi)
+{
+#pragma clang fp contract(off)
+    // rows to blocks and threads as in the CRS and SELL-64 formats (block b
+    // the rows of its chunk of 64-row slices, thread t the rows R0 + t +
+    // 256 j in that order), so the solve kernels' fused dot products round
+    // as with those formats
+    int sb, se;
+    chunk_of((m.n + kSellC - 1) / kSellC, G, b, sb, se);
+    const int beg = sb * kSellC, end = min(se * kSellC, m.n);
+    const int n = m.n, K = KC > 0 ? KC : m.dia_k;
+    const OGSX_GL VT* dv = m.dia_val;
+    // one row per thread at a time: two (the FP32 form's) took 229 VGPRs
+    // for 13 offsets, 2 waves per SIMD, 94 us on 128^3 against 72 us here
+    // (72 VGPRs, 7 waves)
+    constexpr int R = 1;
+    constexpr int kUnroll = KC > 0 ? KC : 4;
+    for (int i = beg + threadIdx.x; i < end; i += R * kBlock) {
+        int r[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) r[q] = min(i + q * kBlock, end - 1);
+        ACC s[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) s[q] = 0;
+#pragma unroll kUnroll
+        for (int k = K - 1; k >= 0; k--) {
+            const int o = m.dia_off[k];
+            const OGSX_GL VT* u = dv + (int64_t)(k + 1) * n;
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                const int lo = max(r[q] - o, 0);
+                const ACC a = r[q] - o >= 0 ? (ACC)u[lo] : (ACC)0;
+                const ACC prod = a * (ACC)x[lo];
+                s[q] = s[q] + prod;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const ACC prod = (ACC)dv[r[q]] * (ACC)x[r[q]];
+            s[q] = s[q] + prod;
+        }
+#pragma unroll kUnroll
+        for (int k = 0; k < K; k++) {
+            const int o = m.dia_off[k];
+            const OGSX_GL VT* u = dv + (int64_t)(k + 1) * n;
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                const int up = min(r[q] + o, n - 1);  // u[i] = 0 where i + o >= n
+                const ACC prod = (ACC)u[r[q]] * (ACC)x[up];
+                s[q] = s[q] + prod;
+            }
+        }
+        epi(r[0], s[0]);
+#pragma unroll
+        for (int q = 1; q < R; q++)
+            if (i + q * kBlock < end) epi(r[q], s[q]);
+    }
+}
+
 // The offset counts of the structured meshes (4: 9-point quads, 7: 15-point
-// tets, 13: 27-point hexes) get fully unrolled instances.
+// tets, 13: 27-point hexes) get fully unrolled instances.  FP64 values are
+// summed in CRS order (spmv_symdia_crs_k), FP32 ones per offset pair.
 template <typename VT, typename XT, typename ACC, typename Epi>
 __device__ __forceinline__ void spmv_symdia(const GMat<VT>& m, const OGSX_GL XT* __restrict__ x,
                                             int G, int b, Epi&& epi)
 {
+    if constexpr (sizeof(VT) == 8) {
+        switch (m.dia_k) {
+            case 4: spmv_symdia_crs_k<4, VT, XT, ACC>(m, x, G, b, epi); break;
+            case 7: spmv_symdia_crs_k<7, VT, XT, ACC>(m, x, G, b, epi); break;
+            case 13: spmv_symdia_crs_k<13, VT, XT, ACC>(m, x, G, b, epi); break;
+            default: spmv_symdia_crs_k<0, VT, XT, ACC>(m, x, G, b, epi); break;
+        }
+        return;
+    }
     switch (m.dia_k) {
         case 4: spmv_symdia_k<4, VT, XT, ACC>(m, x, G, b, epi); break;
         case 7: spmv_symdia_k<7, VT, XT, ACC>(m, x, G, b, epi); break;
@@ -1744,20 +1823,23 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_both_g(Grp g, int tri)
     });
 }
 
-// FP32 symmetric diagonal copy of A (see spmv_symdia) through its position
-// map: one thread per row, its K + 1 entries 8 at a time (map reads and
-// value writes coalesced across the threads; the gathered values of one
-// row share its few cache lines).  Taken position by position, as the
-// slice-layout copies are, every diagonal pass fetched A's lines anew:
-// 561 us against 331 us on 128^3.
-__global__ __launch_bounds__(kBlock) void k_to_dia32_g(Grp g)
+// Symmetric diagonal copy of A (see spmv_symdia; T = float: the FP32
+// matrix, T = double: the FP64 one) through its position map: one thread
+// per row, its K + 1 entries 8 at a time (map reads and value writes
+// coalesced across the threads; the gathered values of one row share its
+// few cache lines).  Taken position by position, as the slice-layout copies
+// are, every diagonal pass fetched A's lines anew: 561 us against 331 us on
+// 128^3 (FP32).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_to_dia_g(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
-    OGSX_GL float* dv = (OGSX_GL float*)gl(S.m32.dia_val);
+    const MatView<T>& M = mv_of<T>(S);
+    OGSX_GL T* dv = (OGSX_GL T*)gl(M.dia_val);
     const OGSX_GL int* src = gl(S.dia_src);
     const OGSX_GL double* A = gl(S.A);
-    const int n = S.n, K1 = S.m32.dia_k + 1;
+    const int n = S.n, K1 = M.dia_k + 1;
     constexpr int U = 8;
     vec_vblocks(c, n, [&](int i) {
         for (int k0 = 0; k0 < K1; k0 += U) {
@@ -1769,7 +1851,7 @@ __global__ __launch_bounds__(kBlock) void k_to_dia32_g(Grp g)
             for (int u = 0; u < U; u++) v[u] = p[u] >= 0 ? A[p[u]] : 0.0;
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (k0 + u < K1) dv[(int64_t)(k0 + u) * n + i] = (float)v[u];
+                if (k0 + u < K1) dv[(int64_t)(k0 + u) * n + i] = (T)v[u];
         }
     });
 }

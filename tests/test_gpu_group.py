@@ -79,7 +79,7 @@ def test_group_shares_formats_within_slack(prec):
             if k == 1:
                 assert f == 5
                 continue
-            if fp32 and f == 25:
+            if f == 25:
                 # the symmetric diagonal copy is chosen by structure (a
                 # symmetric system on a banded pattern), not by the tuner
                 continue

@@ -271,6 +271,32 @@ def test_solve_with_slice_triples(prec, fmt):
     s.close()
 
 
+def test_symmetric_diagonal_fp64_copy_identical_to_crs():
+    """An FP64 system that is symmetric by construction on a banded pattern
+    holds its matrix as the symmetric diagonal copy (format 25), summed in
+    CRS order with the rows on the SELL-64 threads: solves and the
+    stand-alone SpMV equal the plain CRS format's bit for bit (quads: 4
+    offsets, hexes: 13)."""
+    for name in ("quad", "hex"):
+        sc = SCEN[name]
+        v = np.random.default_rng(5).standard_normal(sc.n)
+        out = []
+        for forced in (None, 9):
+            pat, s = W.make_system(sc, ox.FP64)
+            if forced is not None:
+                s.set_spmv_variant(forced)
+            res = W.run_gpu(sc, method=ox.CG, dirichlet=ox.DIRICHLET_SYMMETRIC, keep_steps=True,
+                            system=(pat, s))
+            assert s.spmv_format(False) == (25 if forced is None else 9), name
+            out.append((res, s.spmv(v)))
+            s.close()
+        (dia, y_dia), (crs, y_crs) = out
+        assert [i["iterations"] for i in dia["info"]] == [i["iterations"] for i in crs["info"]]
+        for x, xr in zip(dia["steps"], crs["steps"]):
+            assert np.array_equal(x, xr), name
+        assert np.array_equal(y_dia, y_crs), name
+
+
 def test_symmetric_diagonal_fp32_copy():
     """An FP32-IR system whose matrix is symmetric by construction, on a
     banded pattern, runs its inner FP32 solve on the symmetric diagonal copy
