@@ -184,7 +184,7 @@ struct ogsx_system {
     int* dia_off = nullptr;
     int* dia_src = nullptr;  // CRS position of every stored entry (-1: zero)
     float* dia_val32 = nullptr;   // FP32 copy (FP32-IR systems)
-    double* dia_val64 = nullptr;  // FP64 copy (FP64 systems)
+    double* dia_val64 = nullptr;  // FP64 copy (solves, FP64 residuals)
     int dia_k = 0;
     uint64_t dia32_version = 0, dia64_version = 0;
     int fmt64_tuned = 0;  // the tuner's FP64 format while format 25 stands in
@@ -720,9 +720,10 @@ int64_t system_bytes(const ogsx_pattern* p, int precision)
              b((int64_t)L.toff.size(), 4) + b(L.npad3, 4) + extra * 8 +
              (precision == OGSX_FP32_IR ? extra * 4 : 0);
     }
-    if (L.dia_ok)  // offsets, position map, values (FP32 or FP64)
+    if (L.dia_ok)  // offsets, position map, FP64 values (and FP32 ones for IR)
         t += b((int64_t)L.doff.size(), 4) + b(((int64_t)L.doff.size() + 1) * n, 4) +
-             b(((int64_t)L.doff.size() + 1) * n, precision == OGSX_FP32_IR ? 4 : 8);
+             b(((int64_t)L.doff.size() + 1) * n, 8) +
+             (precision == OGSX_FP32_IR ? b(((int64_t)L.doff.size() + 1) * n, 4) : 0);
     // element data
     t += 3 * b(n, 8) + b(E, 8) + b(E * ed * ed, 8) + b(kMaxGaussPts * 8, 8) +
          b(kMaxGaussPts * 24, 8) + b(kMaxGaussPts * 3, 8);
@@ -1175,8 +1176,8 @@ int ogsx_system_create(const ogsx_pattern* p, int precision, ogsx_system** out)
             const size_t nd = ((size_t)s->dia_k + 1) * (size_t)h.n;
             if ((rc = dalloc(s, &s->dia_off, Ly.doff.size())) ||
                 (rc = dalloc(s, &s->dia_src, Ly.dsrc.size())) ||
-                (rc = precision == OGSX_FP32_IR ? dalloc(s, &s->dia_val32, nd)
-                                                : dalloc(s, &s->dia_val64, nd)))
+                (rc = dalloc(s, &s->dia_val64, nd)) ||
+                (precision == OGSX_FP32_IR && (rc = dalloc(s, &s->dia_val32, nd))))
                 return cleanup(rc);
             if (up(s->dia_off, Ly.doff.data(), 4 * Ly.doff.size()) ||
                 up(s->dia_src, Ly.dsrc.data(), 4 * Ly.dsrc.size()))
