@@ -1829,14 +1829,16 @@ __global__ __launch_bounds__(kBlock) void k_to_sell_both_g(Grp g, int tri)
 // coalesced across the threads; the gathered values of one row share its
 // few cache lines).  Taken position by position, as the slice-layout copies
 // are, every diagonal pass fetched A's lines anew: 561 us against 331 us on
-// 128^3 (FP32).
-template <typename T>
+// 128^3 (FP32).  ALSO32 (with T = double): the FP32 copy written in the
+// same pass, from the same gathered values.
+template <typename T, bool ALSO32 = false>
 __global__ __launch_bounds__(kBlock) void k_to_dia_g(Grp g)
 {
     const Ctx c = ctx_of(g);
     const SysDev& S = *c.S;
     const MatView<T>& M = mv_of<T>(S);
     OGSX_GL T* dv = (OGSX_GL T*)gl(M.dia_val);
+    OGSX_GL float* dv32 = ALSO32 ? (OGSX_GL float*)gl(S.m32.dia_val) : nullptr;
     const OGSX_GL int* src = gl(S.dia_src);
     const OGSX_GL double* A = gl(S.A);
     const int n = S.n, K1 = M.dia_k + 1;
@@ -1851,7 +1853,10 @@ __global__ __launch_bounds__(kBlock) void k_to_dia_g(Grp g)
             for (int u = 0; u < U; u++) v[u] = p[u] >= 0 ? A[p[u]] : 0.0;
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (k0 + u < K1) dv[(int64_t)(k0 + u) * n + i] = (T)v[u];
+                if (k0 + u < K1) {
+                    dv[(int64_t)(k0 + u) * n + i] = (T)v[u];
+                    if constexpr (ALSO32) dv32[(int64_t)(k0 + u) * n + i] = (float)v[u];
+                }
         }
     });
 }
